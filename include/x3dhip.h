@@ -1,0 +1,187 @@
+/*
+ * x3dhip.h -- C ABI of libx3dhip, the MI355X (gfx950) implementation of
+ * x3div's periodic Euler-timestep hot path.
+ *
+ * Conventions
+ *  - Every field is a device pointer to an (n1,n2,n3) array with the first
+ *    index fastest, i.e. the Fortran layout the reference uses in every
+ *    pencil.  At one rank the x-, y- and z-pencils are the same array.
+ *  - Calls are ordered on the context's HIP stream (x3d_set_stream); only
+ *    calls that return host values (reductions, x3d_sync) synchronise.
+ *  - Every entry point returns 0 (X3D_OK) or a negative X3D_ERR_* code; the
+ *    Fortran binding turns a nonzero code into the decomp_2d_abort path the
+ *    reference uses for fatal errors (src/x3d_tools.f90:70-74).
+ *  - Outputs are fully overwritten and inputs are not modified (except the
+ *    in-place Thomas and Poisson calls), as in the reference.
+ *  - Boundary conditions are triply periodic (ncl?1 = ncl?n = 0), the only
+ *    case the reference's Poisson solver accepts (src/poisson.f90:104-108).
+ *  - Suffix _f32 marks the precision form: FP32 fields, FP32 line solves.
+ *
+ * Reference citations are relative to the x3div source tree (src/...).
+ */
+#ifndef X3DHIP_H
+#define X3DHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define X3D_OK 0
+#define X3D_ERR_ARG (-1)         /* bad pointer, id or enum value            */
+#define X3D_ERR_SIZE (-2)        /* sizes do not match the context / n < 8   */
+#define X3D_ERR_HIP (-3)         /* a HIP runtime call failed                */
+#define X3D_ERR_FFT (-4)         /* a hipFFT call failed                     */
+#define X3D_ERR_ALLOC (-5)       /* device allocation failed                 */
+#define X3D_ERR_UNSUPPORTED (-6) /* line length not factorable for the kernels */
+
+/* Operator instances (src/x3d_operator_1d.f90:33-47), in this order. */
+enum {
+  X3D_OP_DERX = 0, X3D_OP_DERXP, X3D_OP_DERY, X3D_OP_DERYP, X3D_OP_DERZ, X3D_OP_DERZP,
+  X3D_OP_DERXVP, X3D_OP_DERYVP, X3D_OP_DERZVP,
+  X3D_OP_DERXPV, X3D_OP_DERYPV, X3D_OP_DERZPV,
+  X3D_OP_INTXVP, X3D_OP_INTYVP, X3D_OP_INTZVP,
+  X3D_OP_INTXPV, X3D_OP_INTYPV, X3D_OP_INTZPV,
+  X3D_NOPS
+};
+
+/* Initial-field kinds for x3d_init_fields. */
+enum {
+  X3D_FIELD_REFERENCE_VEL = 0, /* u=sin cos cos, v=cos sin cos, w=-2 cos cos sin (src/mom.f90:47-52) */
+  X3D_FIELD_TG_NOISE = 1,      /* u=sin cos cos, v=-cos sin cos, w=0, each + amp*(2U-1) */
+  X3D_FIELD_FOURIER8 = 2       /* 8 seeded Fourier modes per component, amplitude 1/8 each */
+};
+
+typedef struct x3d_ctx x3d_ctx;
+
+/* ---- library / context -------------------------------------------------- */
+int         x3d_version(void);
+const char *x3d_strerror(int code);
+
+/* Grid set-up: parameter() with periodic defaults (src/parameters.f90:24-90),
+ * x3d_operator_{x,y,z}_data_init + x3d_operator_1d_init
+ * (src/x3d_tools.f90:178-182, src/x3d_operator_1d.f90:59-195),
+ * x3d_derive_init (src/x3d_derive.f90:66-93) and decomp_2d_poisson_init
+ * (src/poisson.f90:73-231), on the current HIP device. */
+int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_ctx **ctx);
+/* x3d_operator_1d_finalize + decomp_2d_poisson_finalize (src/poisson.f90:237-271) */
+int x3d_destroy(x3d_ctx *ctx);
+int x3d_set_stream(x3d_ctx *ctx, void *hip_stream);
+int x3d_sync(x3d_ctx *ctx);
+/* MPI_Wtime replacement for the step timer (src/xcompact3d.f90:78,98) */
+double x3d_wtime(void);
+int x3d_get_sizes(const x3d_ctx *ctx, int *nx, int *ny, int *nz);
+
+/* Device field memory owned by the library; the Fortran side maps it with
+ * c_f_pointer (the reference's module allocatables, src/var.f90:39-201). */
+int x3d_field_alloc(x3d_ctx *ctx, size_t nbytes, void **dptr);
+int x3d_field_free(x3d_ctx *ctx, void *dptr);
+int x3d_memcpy_h2d(x3d_ctx *ctx, void *dst, const void *src, size_t nbytes);
+int x3d_memcpy_d2h(x3d_ctx *ctx, void *dst, const void *src, size_t nbytes);
+
+/* ---- operator tables (type x3doperator1d, src/x3d_operator_1d.f90:12-30) -- */
+/* Host copies of one operator's f, s, w, periodic vectors (length n) and alfa. */
+int x3d_operator_get(const x3d_ctx *ctx, int op, double *f, double *s, double *w,
+                     double *periodic, double *alfa, int *n, int *npaire);
+/* Replace one operator's tables (drop-in for x3d_operator_1d%init,
+ * src/x3d_operator_1d.f90:234-266, when the caller builds its own). */
+int x3d_operator_set(x3d_ctx *ctx, int op, const double *f, const double *s, const double *w,
+                     const double *periodic, double alfa);
+/* Scheme constants per direction, 11 each: alfai afi bfi alcai aci bci ailcai aici bici cici dici
+ * (module x3d_operator_{x,y,z}_data, src/x3d_operator_x_data.f90:15-18). */
+int x3d_scheme_consts(const x3d_ctx *ctx, double *out33);
+
+/* ---- batched Thomas, in place (src/thomas.f90:15-28 generics) ------------ */
+/* xthomas_0 / ythomas_0 / zthomas_0 (src/thomas.f90:34-58, 87-111, 139-163):
+ * forward/backward sweeps with op's f,s,w then the periodic correction. */
+int x3d_xthomas(x3d_ctx *ctx, double *tt, int op, int nx, int ny, int nz);
+int x3d_ythomas(x3d_ctx *ctx, double *tt, int op, int nx, int ny, int nz);
+int x3d_zthomas(x3d_ctx *ctx, double *tt, int op, int nx, int ny, int nz);
+
+/* ---- collocated first derivatives (procedure pointers derx/dery/derz,
+ *      abstract interfaces src/x3d_derive.f90:18-51; periodic derx_00,
+ *      dery_00, derz_00 at :111-152, :279-339, :503-558) ------------------ */
+int x3d_derx(x3d_ctx *ctx, double *tx, const double *ux, int op, int nx, int ny, int nz);
+/* ppy is the stretching vector of DERIVATIVE_Y; istret = 0 (src/parameters.f90:117)
+ * so it is accepted and unused, as in the reference. May be NULL. */
+int x3d_dery(x3d_ctx *ctx, double *ty, const double *uy, int op, const double *ppy,
+             int nx, int ny, int nz);
+int x3d_derz(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nz);
+
+/* ---- staggered derivatives and interpolations (src/x3d_staggered.f90) ---- */
+int x3d_derxvp(x3d_ctx *ctx, double *tx, const double *ux, int op, int nx, int nxm, int ny, int nz);    /* :23-106  */
+int x3d_interxvp(x3d_ctx *ctx, double *tx, const double *ux, int op, int nx, int nxm, int ny, int nz);  /* :110-217 */
+int x3d_derxpv(x3d_ctx *ctx, double *tx, const double *ux, int op, int nxm, int nx, int ny, int nz);    /* :221-285 */
+int x3d_interxpv(x3d_ctx *ctx, double *tx, const double *ux, int op, int nxm, int nx, int ny, int nz);  /* :289-404 */
+int x3d_interyvp(x3d_ctx *ctx, double *ty, const double *uy, int op, int nx, int ny, int nym, int nz);  /* :408-541 */
+int x3d_deryvp(x3d_ctx *ctx, double *ty, const double *uy, int op, const double *ppyi,
+               int nx, int ny, int nym, int nz);                                                       /* :545-637 */
+int x3d_interypv(x3d_ctx *ctx, double *ty, const double *uy, int op, int nx, int nym, int ny, int nz);  /* :641-786 */
+int x3d_derypv(x3d_ctx *ctx, double *ty, const double *uy, int op, const double *ppy,
+               int nx, int nym, int ny, int nz);                                                       /* :790-883 */
+int x3d_derzvp(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nz, int nzm);    /* :887-997 */
+int x3d_interzvp(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nz, int nzm);  /* :1001-1139 */
+int x3d_derzpv(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nzm, int nz);    /* :1143-1229 */
+int x3d_interzpv(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nzm, int nz);  /* :1233-1383 */
+
+/* ---- pencil transposes (src/x3d_transpose.f90:16-47 generics; 2decomp's
+ *      transpose_* called directly at src/navier.f90:171-173,189-190).
+ *      At one rank every pencil holds the whole array in the same order, so
+ *      each is a device copy of n1*n2*n3 elements of elem_bytes (8 real,
+ *      16 complex FP64); in == out is a no-op. ---------------------------- */
+int x3d_transpose_x_to_y(x3d_ctx *ctx, const void *in, void *out, size_t elem_bytes, int n1, int n2, int n3);
+int x3d_transpose_y_to_z(x3d_ctx *ctx, const void *in, void *out, size_t elem_bytes, int n1, int n2, int n3);
+int x3d_transpose_z_to_y(x3d_ctx *ctx, const void *in, void *out, size_t elem_bytes, int n1, int n2, int n3);
+int x3d_transpose_y_to_x(x3d_ctx *ctx, const void *in, void *out, size_t elem_bytes, int n1, int n2, int n3);
+
+/* ---- spectral Poisson (procedure pointer poisson => poisson_000,
+ *      src/poisson.f90:57-63, 276-419): r2c FFT, division by the modified
+ *      wavenumber with null modes zeroed, c2r FFT; in place on rhs. -------- */
+int x3d_poisson(x3d_ctx *ctx, double *rhs);
+/* Host copies of the public module data ax,bx,ay,by,az,bz (src/poisson.f90:40-42)
+ * and of kxyz's real part on the reference's (nx, ny, nz/2+1) spectral grid
+ * (src/poisson.f90:45, 660-699).  Any pointer may be NULL. */
+int x3d_poisson_tables(const x3d_ctx *ctx, double *ax, double *bx, double *ay, double *by,
+                       double *az, double *bz, double *kxyz_re);
+
+/* ---- timestep pieces (callers of the operators) ----------------------- */
+/* calculate_transeq_rhs / momentum_rhs_eq (src/transeq.f90:18-355) */
+int x3d_momentum_rhs(x3d_ctx *ctx, double *dux, double *duy, double *duz,
+                     const double *ux, const double *uy, const double *uz);
+/* int_time / intt, Euler: u += gdt*du (src/time_integrators.f90:20-106) */
+int x3d_int_time(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
+                 const double *dux, const double *duy, const double *duz);
+/* divergence, nlock = 1 (src/navier.f90:98-236) */
+int x3d_divergence(x3d_ctx *ctx, double *pp3, const double *ux, const double *uy, const double *uz);
+/* gradp (src/navier.f90:249-359) */
+int x3d_gradp(x3d_ctx *ctx, double *px, double *py, double *pz, const double *pp3);
+/* cor_vel (src/navier.f90:64-91) */
+int x3d_cor_vel(x3d_ctx *ctx, double *ux, double *uy, double *uz,
+                const double *px, const double *py, const double *pz);
+/* solve_poisson = divergence + poisson + gradp (src/navier.f90:21-56) */
+int x3d_solve_poisson(x3d_ctx *ctx, double *pp3, double *px, double *py, double *pz,
+                      const double *ux, const double *uy, const double *uz);
+/* One iteration of the step loop (src/xcompact3d.f90:80-95): transeq, Euler
+ * update, solve_poisson, cor_vel, with the elementwise glue fused into the
+ * line kernels.  Same results as the four calls above in sequence. */
+int x3d_step(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
+             double *dux, double *duy, double *duz, double *pp3,
+             double *px, double *py, double *pz);
+
+/* ---- test mode and fields --------------------------------------------- */
+/* test_du/test_dv/test_dw RMS error vs the analytic derivative of vel()
+ * (src/mom.f90:59-243); which = 0,1,2.  Synchronises. */
+int x3d_test_err(x3d_ctx *ctx, const double *d, int which, double *err);
+/* "DIV U* max mean" statistics (src/navier.f90:213-231).  Synchronises. */
+int x3d_div_stats(x3d_ctx *ctx, const double *pp3, double *tmax, double *tmean);
+/* Initial fields on the device (case::init -> vel, src/case.f90:20-58 and
+ * src/mom.f90:26-57 for kind 0; seeded benchmark fields for kinds 1, 2). */
+int x3d_init_fields(x3d_ctx *ctx, int kind, uint64_t seed, double amp,
+                    double *ux, double *uy, double *uz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* X3DHIP_H */

@@ -49,6 +49,8 @@ def lib():
         L.orc_delete.argtypes = [vp]
         L.orc_get_op.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_get_consts.argtypes = [vp, dp]
+        L.orc_get_kxyz.argtypes = [vp, dp, dp]
+        L.orc_get_ab.argtypes = [vp, dp]
         L.orc_apply.argtypes = [vp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_apply.restype = C.c_int
         L.orc_thomas.argtypes = [vp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -121,6 +123,20 @@ class Oracle:
         lib().orc_get_consts(self._c, _p(out))
         names = ["alfai", "afi", "bfi", "alcai", "aci", "bci", "ailcai", "aici", "bici", "cici", "dici"]
         return [dict(zip(names, out[11 * d:11 * d + 11])) for d in range(3)]
+
+    def kxyz(self):
+        """(Re, Im) of kxyz on the (nx, ny, nz/2+1) spectral grid."""
+        re = np.zeros((self.nz // 2 + 1, self.ny, self.nx)); im = np.zeros_like(re)
+        lib().orc_get_kxyz(self._c, _p(re), _p(im))
+        return re, im
+
+    def ab(self):
+        """ax, bx, ay, by, az, bz of abxyz."""
+        out = np.zeros(2 * (self.nx + self.ny + self.nz))
+        lib().orc_get_ab(self._c, _p(out))
+        sizes = [self.nx, self.nx, self.ny, self.ny, self.nz, self.nz]
+        parts = np.split(out, np.cumsum(sizes)[:-1])
+        return dict(zip(["ax", "bx", "ay", "by", "az", "bz"], parts))
 
     def apply(self, name: str, u: np.ndarray, kind: str | None = None) -> np.ndarray:
         """Apply one operator (der?/der?vp/der?pv/inter?vp/inter?pv) along its axis."""

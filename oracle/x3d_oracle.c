@@ -286,6 +286,22 @@ void orc_get_consts(const orc_ctx *c, double *out)
   }
 }
 
+void orc_get_kxyz(const orc_ctx *c, double *re, double *im)
+{
+  size_t ns = (size_t)c->nx * c->ny * (c->nz / 2 + 1);
+  for (size_t q = 0; q < ns; ++q) { re[q] = c->kxyz[2 * q]; im[q] = c->kxyz[2 * q + 1]; }
+}
+
+void orc_get_ab(const orc_ctx *c, double *out)
+{
+  memcpy(out, c->ax, sizeof(double) * c->nx); out += c->nx;
+  memcpy(out, c->bx, sizeof(double) * c->nx); out += c->nx;
+  memcpy(out, c->ay, sizeof(double) * c->ny); out += c->ny;
+  memcpy(out, c->by, sizeof(double) * c->ny); out += c->ny;
+  memcpy(out, c->az, sizeof(double) * c->nz); out += c->nz;
+  memcpy(out, c->bz, sizeof(double) * c->nz);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Batched Thomas: src/thomas.f90:34-188                                     */
 /* ------------------------------------------------------------------------ */

@@ -62,6 +62,10 @@ void orc_thomas1d(double *tt, const double *ff, const double *fs, const double *
 int  orc_get_op(const orc_ctx *c, int op, double *f, double *s, double *w,
                 double *periodic, double *alfa, int *n, int *npaire);
 void orc_get_consts(const orc_ctx *c, double *out /* 3 x 11 */);
+/* real and imaginary parts of kxyz, each (nx, ny, nz/2+1) */
+void orc_get_kxyz(const orc_ctx *c, double *re, double *im);
+/* ax, bx, ay, by, az, bz concatenated */
+void orc_get_ab(const orc_ctx *c, double *out);
 
 /* ---- batched Thomas (in place) -------------------------------------------- */
 void orc_xthomas_0(double *tt, const orc_op *op, int nx, int ny, int nz);

@@ -1,0 +1,284 @@
+"""Host-side mirror of x3div's hot-path interfaces over libx3dhip.
+
+One :class:`X3D` object is one periodic grid on the current HIP device: the
+reference's ``init_xcompact3d`` numerics set-up (operators, derivative
+pointers, Poisson tables).  Its methods keep the reference's subroutine names
+and argument meaning (output first, then input, then the operator instance)
+and operate on device tensors of shape ``(nz, ny, nx)`` -- the Fortran
+``(nx, ny, nz)`` layout with x fastest.  torch provides the device memory and
+the stream; every computation runs in the library's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+# operator instances, src/x3d_operator_1d.f90:33-47 (x3d_op_<name>)
+OPS = ["derx", "derxp", "dery", "deryp", "derz", "derzp",
+       "derxvp", "deryvp", "derzvp", "derxpv", "derypv", "derzpv",
+       "intxvp", "intyvp", "intzvp", "intxpv", "intypv", "intzpv"]
+OP = {name: i for i, name in enumerate(OPS)}
+
+FIELD_REFERENCE_VEL, FIELD_TG_NOISE, FIELD_FOURIER8 = 0, 1, 2
+
+
+def _op_id(op) -> int:
+    if isinstance(op, str):
+        return OP[op.removeprefix("x3d_op_")]
+    return int(op)
+
+
+def _ptr(t, shape=None) -> int:
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor on the GPU")
+    if not t.is_cuda:
+        raise ValueError("x3div_amd operates on device tensors only (no CPU fallback)")
+    if t.dtype != torch.float64:
+        raise TypeError(f"expected float64, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"shape {tuple(t.shape)} != {tuple(shape)}")
+    return t.data_ptr()
+
+
+class X3D:
+    """A periodic nx*ny*nz grid of lengths (xlx, yly, zlz) on the GPU."""
+
+    def __init__(self, nx: int, ny: int, nz: int, xlx: float = 1.0, yly: float = 1.0,
+                 zlz: float = 1.0, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("x3div_amd needs a HIP device (no CPU fallback)")
+        self.lib = _lib.load()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.L = (xlx, yly, zlz)
+        h = C.c_void_p()
+        check("x3d_create", self.lib.x3d_create(nx, ny, nz, xlx, yly, zlz, C.byref(h)))
+        self._h = h
+        self.set_stream(stream if stream is not None else torch.cuda.current_stream())
+
+    # ---- lifetime / plumbing ----------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.x3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        self.stream = stream
+        check("x3d_set_stream", self.lib.x3d_set_stream(self._h, C.c_void_p(stream.cuda_stream)))
+
+    def sync(self):
+        check("x3d_sync", self.lib.x3d_sync(self._h))
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def field(self, fill: float | None = 0.0):
+        import torch
+        t = torch.empty(self.shape, dtype=torch.float64, device="cuda")
+        if fill is not None:
+            t.fill_(fill)
+        return t
+
+    def _f(self, t):
+        return _ptr(t, self.shape)
+
+    # ---- operator tables ----------------------------------------------------
+    def operator(self, op):
+        """(f, s, w, periodic, alfa, npaire) of one x3doperator1d instance."""
+        o = _op_id(op)
+        n = {0: self.nx, 1: self.ny, 2: self.nz}[_axis(OPS[o])]
+        arrs = [np.zeros(n) for _ in range(4)]
+        alfa = C.c_double(); nn = C.c_int(); npaire = C.c_int()
+        hp = C.POINTER(C.c_double)
+        check("x3d_operator_get", self.lib.x3d_operator_get(
+            self._h, o, *(a.ctypes.data_as(hp) for a in arrs), C.byref(alfa), C.byref(nn),
+            C.byref(npaire)))
+        return (*arrs, alfa.value, npaire.value)
+
+    def set_operator(self, op, f, s, w, periodic, alfa):
+        hp = C.POINTER(C.c_double)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (f, s, w, periodic)]
+        check("x3d_operator_set", self.lib.x3d_operator_set(
+            self._h, _op_id(op), *(a.ctypes.data_as(hp) for a in arrs), float(alfa)))
+
+    def scheme_consts(self):
+        out = np.zeros(33)
+        check("x3d_scheme_consts", self.lib.x3d_scheme_consts(
+            self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        names = ["alfai", "afi", "bfi", "alcai", "aci", "bci", "ailcai", "aici", "bici", "cici", "dici"]
+        return [dict(zip(names, out[11 * d:11 * d + 11])) for d in range(3)]
+
+    # ---- Thomas (src/thomas.f90) ------------------------------------------------
+    def xthomas(self, tt, op):
+        check("x3d_xthomas", self.lib.x3d_xthomas(self._h, self._f(tt), _op_id(op), self.nx, self.ny, self.nz))
+
+    def ythomas(self, tt, op):
+        check("x3d_ythomas", self.lib.x3d_ythomas(self._h, self._f(tt), _op_id(op), self.nx, self.ny, self.nz))
+
+    def zthomas(self, tt, op):
+        check("x3d_zthomas", self.lib.x3d_zthomas(self._h, self._f(tt), _op_id(op), self.nx, self.ny, self.nz))
+
+    # ---- derivatives (src/x3d_derive.f90) ----------------------------------------
+    def derx(self, tx, ux, op="derx"):
+        check("x3d_derx", self.lib.x3d_derx(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.ny, self.nz))
+
+    def dery(self, ty, uy, op="dery", ppy=None):
+        check("x3d_dery", self.lib.x3d_dery(self._h, self._f(ty), self._f(uy), _op_id(op), None,
+                                            self.nx, self.ny, self.nz))
+
+    def derz(self, tz, uz, op="derz"):
+        check("x3d_derz", self.lib.x3d_derz(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz))
+
+    # ---- staggered (src/x3d_staggered.f90) -----------------------------------------
+    def derxvp(self, tx, ux, op="derxvp"):
+        check("x3d_derxvp", self.lib.x3d_derxvp(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.nx, self.ny, self.nz))
+
+    def interxvp(self, tx, ux, op="intxvp"):
+        check("x3d_interxvp", self.lib.x3d_interxvp(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.nx, self.ny, self.nz))
+
+    def derxpv(self, tx, ux, op="derxpv"):
+        check("x3d_derxpv", self.lib.x3d_derxpv(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.nx, self.ny, self.nz))
+
+    def interxpv(self, tx, ux, op="intxpv"):
+        check("x3d_interxpv", self.lib.x3d_interxpv(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.nx, self.ny, self.nz))
+
+    def interyvp(self, ty, uy, op="intyvp"):
+        check("x3d_interyvp", self.lib.x3d_interyvp(self._h, self._f(ty), self._f(uy), _op_id(op), self.nx, self.ny, self.ny, self.nz))
+
+    def deryvp(self, ty, uy, op="deryvp", ppyi=None):
+        check("x3d_deryvp", self.lib.x3d_deryvp(self._h, self._f(ty), self._f(uy), _op_id(op), None, self.nx, self.ny, self.ny, self.nz))
+
+    def interypv(self, ty, uy, op="intypv"):
+        check("x3d_interypv", self.lib.x3d_interypv(self._h, self._f(ty), self._f(uy), _op_id(op), self.nx, self.ny, self.ny, self.nz))
+
+    def derypv(self, ty, uy, op="derypv", ppy=None):
+        check("x3d_derypv", self.lib.x3d_derypv(self._h, self._f(ty), self._f(uy), _op_id(op), None, self.nx, self.ny, self.ny, self.nz))
+
+    def derzvp(self, tz, uz, op="derzvp"):
+        check("x3d_derzvp", self.lib.x3d_derzvp(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz, self.nz))
+
+    def interzvp(self, tz, uz, op="intzvp"):
+        check("x3d_interzvp", self.lib.x3d_interzvp(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz, self.nz))
+
+    def derzpv(self, tz, uz, op="derzpv"):
+        check("x3d_derzpv", self.lib.x3d_derzpv(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz, self.nz))
+
+    def interzpv(self, tz, uz, op="intzpv"):
+        check("x3d_interzpv", self.lib.x3d_interzpv(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz, self.nz))
+
+    def apply(self, name: str, out, inp):
+        """Apply an operator by its instance name with the reference's routine."""
+        routine = {
+            "derx": self.derx, "derxp": self.derx, "dery": self.dery, "deryp": self.dery,
+            "derz": self.derz, "derzp": self.derz, "derxvp": self.derxvp, "deryvp": self.deryvp,
+            "derzvp": self.derzvp, "derxpv": self.derxpv, "derypv": self.derypv, "derzpv": self.derzpv,
+            "intxvp": self.interxvp, "intyvp": self.interyvp, "intzvp": self.interzvp,
+            "intxpv": self.interxpv, "intypv": self.interypv, "intzpv": self.interzpv,
+        }[name]
+        routine(out, inp, name)
+
+    # ---- transposes (src/x3d_transpose.f90) ------------------------------------------
+    def _transpose(self, fn, a, b):
+        eb = a.element_size()
+        if b.element_size() != eb or a.numel() != b.numel() or not (a.is_cuda and b.is_cuda):
+            raise ValueError("transpose: mismatched arrays")
+        check(fn, getattr(self.lib, fn)(self._h, a.data_ptr(), b.data_ptr(), eb, self.nx, self.ny,
+                                        a.numel() // (self.nx * self.ny)))
+
+    def transpose_x_to_y(self, a, b):
+        self._transpose("x3d_transpose_x_to_y", a, b)
+
+    def transpose_y_to_z(self, a, b):
+        self._transpose("x3d_transpose_y_to_z", a, b)
+
+    def transpose_z_to_y(self, a, b):
+        self._transpose("x3d_transpose_z_to_y", a, b)
+
+    def transpose_y_to_x(self, a, b):
+        self._transpose("x3d_transpose_y_to_x", a, b)
+
+    # ---- Poisson (src/poisson.f90) -----------------------------------------------------
+    def poisson(self, rhs):
+        check("x3d_poisson", self.lib.x3d_poisson(self._h, self._f(rhs)))
+
+    def poisson_tables(self):
+        hp = C.POINTER(C.c_double)
+        a = [np.zeros(n) for n in (self.nx, self.nx, self.ny, self.ny, self.nz, self.nz)]
+        k = np.zeros((self.nz // 2 + 1, self.ny, self.nx))
+        check("x3d_poisson_tables", self.lib.x3d_poisson_tables(
+            self._h, *(x.ctypes.data_as(hp) for x in a), k.ctypes.data_as(hp)))
+        names = ["ax", "bx", "ay", "by", "az", "bz"]
+        out = dict(zip(names, a))
+        out["kxyz"] = k
+        return out
+
+    # ---- timestep (src/transeq.f90, navier.f90, time_integrators.f90) -----------------
+    def calculate_transeq_rhs(self, dux, duy, duz, ux, uy, uz):
+        check("x3d_momentum_rhs", self.lib.x3d_momentum_rhs(
+            self._h, *(self._f(t) for t in (dux, duy, duz, ux, uy, uz))))
+
+    momentum_rhs_eq = calculate_transeq_rhs
+
+    def int_time(self, ux, uy, uz, dux, duy, duz, gdt: float = 0.0):
+        check("x3d_int_time", self.lib.x3d_int_time(
+            self._h, float(gdt), *(self._f(t) for t in (ux, uy, uz, dux, duy, duz))))
+
+    def divergence(self, pp3, ux, uy, uz):
+        check("x3d_divergence", self.lib.x3d_divergence(self._h, *(self._f(t) for t in (pp3, ux, uy, uz))))
+
+    def gradp(self, px, py, pz, pp3):
+        check("x3d_gradp", self.lib.x3d_gradp(self._h, *(self._f(t) for t in (px, py, pz, pp3))))
+
+    def cor_vel(self, ux, uy, uz, px, py, pz):
+        check("x3d_cor_vel", self.lib.x3d_cor_vel(self._h, *(self._f(t) for t in (ux, uy, uz, px, py, pz))))
+
+    def solve_poisson(self, pp3, px, py, pz, ux, uy, uz):
+        check("x3d_solve_poisson", self.lib.x3d_solve_poisson(
+            self._h, *(self._f(t) for t in (pp3, px, py, pz, ux, uy, uz))))
+
+    def step(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """One iteration of the reference's step loop (src/xcompact3d.f90:80-95)."""
+        check("x3d_step", self.lib.x3d_step(
+            self._h, float(gdt), *(self._f(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
+
+    # ---- test mode / fields (src/mom.f90, src/navier.f90:213-231) --------------------
+    def test_err(self, d, which: int) -> float:
+        e = C.c_double()
+        check("x3d_test_err", self.lib.x3d_test_err(self._h, self._f(d), which, C.byref(e)))
+        return e.value
+
+    def test_du(self, du):
+        return self.test_err(du, 0)
+
+    def test_dv(self, dv):
+        return self.test_err(dv, 1)
+
+    def test_dw(self, dw):
+        return self.test_err(dw, 2)
+
+    def div_stats(self, pp3):
+        mx = C.c_double(); mean = C.c_double()
+        check("x3d_div_stats", self.lib.x3d_div_stats(self._h, self._f(pp3), C.byref(mx), C.byref(mean)))
+        return mx.value, mean.value
+
+    def init_fields(self, ux, uy, uz, kind: int = FIELD_TG_NOISE, seed: int = 1234, amp: float = 1e-3):
+        check("x3d_init_fields", self.lib.x3d_init_fields(
+            self._h, kind, seed, amp, *(self._f(t) for t in (ux, uy, uz))))
+
+
+def _axis(name: str) -> int:
+    return "xyz".index(name[3])

@@ -1,0 +1,763 @@
+// C-ABI entry points of libx3dhip: context and operator-table set-up,
+// derivative / staggered / Thomas / transpose entries, the step pieces and
+// test-mode reductions.
+#include <sys/time.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "x3d_internal.h"
+
+namespace x3d {
+
+int hip_status(hipError_t e)
+{
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return X3D_ERR_ALLOC;
+  return X3D_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// Operator coefficients (src/x3d_operator_1d.f90).
+// ---------------------------------------------------------------------------
+
+// prepare(b, c, f, s, w, n) (:301-323): forward multipliers s and inverted
+// pivots w of the tridiagonal matrix with lower b, diagonal c, upper f.
+static void prepare(const std::vector<double>& b, const std::vector<double>& cd,
+                    const std::vector<double>& f, std::vector<double>& s, std::vector<double>& w)
+{
+  const size_t n = cd.size();
+  s.assign(n, 0.0);
+  w = cd;
+  for (size_t i = 1; i < n; ++i) {
+    s[i] = b[i - 1] / w[i - 1];
+    w[i] = w[i] - f[i - 1] * s[i];
+  }
+  for (size_t i = 0; i < n; ++i) w[i] = 1.0 / w[i];
+}
+
+// Periodic rows of first_deriv_imp_ (:414-471) and interpol_imp_ (:576-599):
+// off-diagonals a except the last row/column, diagonal (2, 1, ..., 1, 1+a^2).
+static void periodic_rows(double a, int n, std::vector<double>& f, std::vector<double>& cd,
+                          std::vector<double>& b)
+{
+  f.assign(n, 0.0); cd.assign(n, 0.0); b.assign(n, 0.0);
+  f[0] = a; f[1] = a; cd[0] = 2.0; cd[1] = 1.0; b[0] = a; b[1] = a;
+  f[n - 3] = a; f[n - 2] = a; f[n - 1] = 0.0;
+  cd[n - 3] = 1.0; cd[n - 2] = 1.0; cd[n - 1] = 1.0 + a * a;
+  b[n - 3] = a; b[n - 2] = a; b[n - 1] = 0.0;
+  for (int i = 2; i <= n - 4; ++i) { f[i] = a; cd[i] = 1.0; b[i] = a; }
+}
+
+// x3doperator1d%init (:234-266): periodic = thomas1d((-1, 0, ..., 0, alfa)).
+static void op_init(OpHost& o, const std::vector<double>& f, const std::vector<double>& s,
+                    const std::vector<double>& w, double alfa, int paire)
+{
+  const int n = (int)f.size();
+  o.n = n; o.npaire = paire; o.alfa = alfa; o.f = f; o.s = s; o.w = w;
+  o.p.assign(n, 0.0);
+  o.p[0] = -1.0;
+  o.p[n - 1] = alfa;
+  for (int k = 1; k < n; ++k) o.p[k] = o.p[k] - o.p[k - 1] * s[k];
+  o.p[n - 1] = o.p[n - 1] * w[n - 1];
+  for (int k = n - 2; k >= 0; --k) o.p[k] = (o.p[k] - f[k] * o.p[k + 1]) * w[k];
+}
+
+static void scheme(Scheme& s, double h)
+{
+  // first_deriv_exp_ ifirstder = 4 (:350-353)
+  s.alfai = 1.0 / 3.0;
+  s.afi = (7.0 / 9.0) / h;
+  s.bfi = (1.0 / 36.0) / h;
+  // interpol_exp_ ifirstder = 4, ipinter = 3 (:509-545)
+  s.alcai = 9.0 / 62.0;
+  s.aci = (63.0 / 62.0) / h;
+  s.bci = (17.0 / 62.0) / 3.0 / h;
+  s.ailcai = 0.49;
+  s.aici = 1.0 / 128.0 * (75.0 + 70.0 * s.ailcai);
+  s.bici = 1.0 / 256.0 * (126.0 * s.ailcai - 25.0);
+  s.cici = 1.0 / 256.0 * (-10.0 * s.ailcai + 3.0);
+  s.dici = 0.0;
+}
+
+// The six operators of one direction (x3d_operator_1d_init, :76-189, periodic).
+static void direction_ops(x3d_ctx* c, int d, int n)
+{
+  const Scheme& sc = c->sc[d];
+  std::vector<double> f, cd, b, s, w;
+  periodic_rows(sc.alfai, n, f, cd, b);
+  prepare(b, cd, f, s, w);
+  op_init(c->op[X3D_OP_DERX + 2 * d], f, s, w, sc.alfai, 0);
+  op_init(c->op[X3D_OP_DERXP + 2 * d], f, s, w, sc.alfai, 1);  // ffp = ff when ncl1 /= 1
+  periodic_rows(sc.alcai, n, f, cd, b);
+  prepare(b, cd, f, s, w);
+  op_init(c->op[X3D_OP_DERXVP + d], f, s, w, sc.alcai, 1);     // cfx6, csx6, cwx6
+  op_init(c->op[X3D_OP_DERXPV + d], f, s, w, sc.alcai, 0);
+  periodic_rows(sc.ailcai, n, f, cd, b);
+  prepare(b, cd, f, s, w);
+  op_init(c->op[X3D_OP_INTXVP + d], f, s, w, sc.ailcai, 1);    // cifxp6 == cifx6 here
+  op_init(c->op[X3D_OP_INTXPV + d], f, s, w, sc.ailcai, 1);
+}
+
+static int upload_tables(x3d_ctx* c)
+{
+  size_t len = 0;
+  for (int o = 0; o < X3D_NOPS; ++o) { c->off[o] = len; len += 4 * (size_t)c->op[o].n; }
+  if (!c->tab64 || c->tab_len != len) {
+    if (c->tab64) (void)hipFree(c->tab64);
+    if (c->tab32) (void)hipFree(c->tab32);
+    c->tab64 = nullptr; c->tab32 = nullptr;
+    X3D_CHECK_HIP(hipMalloc(&c->tab64, len * sizeof(double)));
+    X3D_CHECK_HIP(hipMalloc(&c->tab32, len * sizeof(float)));
+    c->tab_len = len;
+  }
+  std::vector<double> h(len);
+  std::vector<float> hf(len);
+  for (int o = 0; o < X3D_NOPS; ++o) {
+    const OpHost& op = c->op[o];
+    const size_t n = op.n;
+    std::memcpy(&h[c->off[o]], op.f.data(), n * sizeof(double));
+    std::memcpy(&h[c->off[o] + n], op.s.data(), n * sizeof(double));
+    std::memcpy(&h[c->off[o] + 2 * n], op.w.data(), n * sizeof(double));
+    std::memcpy(&h[c->off[o] + 3 * n], op.p.data(), n * sizeof(double));
+  }
+  for (size_t i = 0; i < len; ++i) hf[i] = (float)h[i];
+  X3D_CHECK_HIP(hipMemcpy(c->tab64, h.data(), len * sizeof(double), hipMemcpyHostToDevice));
+  X3D_CHECK_HIP(hipMemcpy(c->tab32, hf.data(), len * sizeof(float), hipMemcpyHostToDevice));
+  return X3D_OK;
+}
+
+int ensure_work(x3d_ctx* c)
+{
+  const size_t N = (size_t)c->nx * c->ny * c->nz;
+  if (c->work[0] && c->work_elems == N) return X3D_OK;
+  for (double*& w : c->work) {
+    if (w) (void)hipFree(w);
+    w = nullptr;
+  }
+  for (double*& w : c->work) X3D_CHECK_HIP(hipMalloc(&w, N * sizeof(double)));
+  c->work_elems = N;
+  return X3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise glue kernels.
+// ---------------------------------------------------------------------------
+enum EwOp { EW_MUL = 0, EW_FMA_ADD = 1, EW_ADD = 2, EW_AXPY = 3, EW_SUB = 4 };
+
+// EW_MUL:     o = a * b
+// EW_FMA_ADD: o = t + a * b         (src/transeq.f90:114, 133, 152)
+// EW_ADD:     o = a + b             (src/transeq.f90:164, 176; navier.f90:183, 201)
+// EW_AXPY:    o = g * a + b         (src/time_integrators.f90:42)
+// EW_SUB:     o = a - b             (src/navier.f90:84-86)
+__global__ void k_ew(int opk, double* __restrict__ o, const double* a, const double* b,
+                     const double* t, double gsc, long long N)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += stride) {
+    double r;
+    switch (opk) {
+      case EW_MUL: r = a[q] * b[q]; break;
+      case EW_FMA_ADD: r = t[q] + a[q] * b[q]; break;
+      case EW_ADD: r = a[q] + b[q]; break;
+      case EW_AXPY: r = gsc * a[q] + b[q]; break;
+      default: r = a[q] - b[q]; break;
+    }
+    o[q] = r;
+  }
+}
+
+static int ew(x3d_ctx* c, int opk, double* o, const double* a, const double* b, const double* t,
+              double gsc)
+{
+  const long long N = (long long)c->nx * c->ny * c->nz;
+  long long blocks = (N + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_ew, dim3((unsigned)blocks), dim3(256), 0, c->stream, opk, o, a, b, t, gsc, N);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (deterministic two-level sums in FP64).
+// ---------------------------------------------------------------------------
+constexpr int RED_BLOCKS = 1024;
+
+__device__ inline uint64_t splitmix64(uint64_t x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ inline double uniform01(uint64_t seed, int comp, uint64_t idx)
+{
+  const uint64_t key = splitmix64(seed ^ ((uint64_t)(comp + 1) * 0x9E3779B97F4A7C15ull));
+  return (double)(splitmix64(key + idx) >> 11) * 0x1.0p-53;
+}
+
+struct GridInfo {
+  int nx, ny, nz;
+  double dx, dy, dz, xlx, yly, zlz;
+};
+
+// which: 0 dudx, 1 dvdy, 2 dwdz (src/mom.f90:95-96, 158-159, 217-218);
+// 3 = max/sum of |x| (src/navier.f90:215-217).  Partial results per block.
+__global__ void k_reduce_partial(const double* __restrict__ d, int which, GridInfo g,
+                                 double* __restrict__ part)
+{
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  double acc = 0.0, mx = 0.0;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < N;
+       q += (long long)gridDim.x * blockDim.x) {
+    if (which == 3) {
+      const double a = fabs(d[q]);
+      acc += a;
+      mx = a > mx ? a : mx;
+    } else {
+      const int i = (int)(q % g.nx);
+      const int j = (int)((q / g.nx) % g.ny);
+      const int k = (int)(q / ((long long)g.nx * g.ny));
+      const double x = i * g.dx, y = j * g.dy, z = k * g.dz;
+      double ana = cos(twopi * (x / g.xlx)) * cos(twopi * (y / g.yly)) * cos(twopi * (z / g.zlz));
+      if (which == 0) ana = twopi * ana / g.xlx;
+      else if (which == 1) ana = twopi * ana / g.yly;
+      else ana = twopi * (-2.0 * ana) / g.zlz;
+      const double e = d[q] - ana;
+      acc += e * e;
+    }
+  }
+  __shared__ double ssum[256], smax[256];
+  ssum[threadIdx.x] = acc;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      ssum[threadIdx.x] += ssum[threadIdx.x + s];
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = ssum[0]; part[2 * blockIdx.x + 1] = smax[0]; }
+}
+
+__global__ void k_reduce_final(const double* __restrict__ part, int nparts, double* __restrict__ res)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0, m = 0.0;
+    for (int b = 0; b < nparts; ++b) { s += part[2 * b]; m = fmax(m, part[2 * b + 1]); }
+    res[0] = s; res[1] = m;
+  }
+}
+
+static int reduce(x3d_ctx* c, const double* d, int which, double* sum, double* mx)
+{
+  if (!c->red) X3D_CHECK_HIP(hipMalloc(&c->red, sizeof(double) * (2 * RED_BLOCKS + 2)));
+  GridInfo g{c->nx, c->ny, c->nz, c->dx, c->dy, c->dz, c->xlx, c->yly, c->zlz};
+  hipLaunchKernelGGL(k_reduce_partial, dim3(RED_BLOCKS), dim3(256), 0, c->stream, d, which, g, c->red);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, c->stream, c->red, RED_BLOCKS,
+                     c->red + 2 * RED_BLOCKS);
+  X3D_CHECK_HIP(hipGetLastError());
+  double res[2];
+  X3D_CHECK_HIP(hipMemcpyAsync(res, c->red + 2 * RED_BLOCKS, sizeof(res), hipMemcpyDeviceToHost, c->stream));
+  X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  *sum = res[0];
+  *mx = res[1];
+  return X3D_OK;
+}
+
+// Initial fields; same integer generator as the host reference so both sides
+// draw identical noise.
+__global__ void k_init_fields(int kind, uint64_t seed, double amp, GridInfo g, const int* kv,
+                              const double* ph, double* __restrict__ ux, double* __restrict__ uy,
+                              double* __restrict__ uz)
+{
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const long long N = (long long)g.nx * g.ny * g.nz;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < N;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % g.nx);
+    const int j = (int)((q / g.nx) % g.ny);
+    const int k = (int)(q / ((long long)g.nx * g.ny));
+    if (kind == 2) {
+      double* dst[3] = {ux, uy, uz};
+      for (int cpt = 0; cpt < 3; ++cpt) {
+        double acc = 0.0;
+        for (int m = 0; m < 8; ++m) {
+          const int* kk = kv + 3 * (8 * cpt + m);
+          const double arg = twopi * ((double)kk[0] * (double)i / (double)g.nx +
+                                      (double)kk[1] * (double)j / (double)g.ny +
+                                      (double)kk[2] * (double)k / (double)g.nz) + ph[8 * cpt + m];
+          acc += 0.125 * sin(arg);
+        }
+        dst[cpt][q] = acc;
+      }
+      continue;
+    }
+    const double x = i * g.dx, y = j * g.dy, z = k * g.dz;
+    const double sx = sin(twopi * (x / g.xlx)), cx = cos(twopi * (x / g.xlx));
+    const double sy = sin(twopi * (y / g.yly)), cy = cos(twopi * (y / g.yly));
+    const double sz = sin(twopi * (z / g.zlz)), cz = cos(twopi * (z / g.zlz));
+    if (kind == 0) {
+      ux[q] = sx * cy * cz;
+      uy[q] = cx * sy * cz;
+      uz[q] = -2.0 * cx * cy * sz;
+    } else {
+      ux[q] = sx * cy * cz + amp * (2.0 * uniform01(seed, 0, q) - 1.0);
+      uy[q] = -cx * sy * cz + amp * (2.0 * uniform01(seed, 1, q) - 1.0);
+      uz[q] = amp * (2.0 * uniform01(seed, 2, q) - 1.0);
+    }
+  }
+}
+
+static uint64_t host_splitmix64(uint64_t x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+static double host_uniform01(uint64_t seed, int comp, uint64_t idx)
+{
+  const uint64_t key = host_splitmix64(seed ^ ((uint64_t)(comp + 1) * 0x9E3779B97F4A7C15ull));
+  return (double)(host_splitmix64(key + idx) >> 11) * 0x1.0p-53;
+}
+
+}  // namespace x3d
+
+using namespace x3d;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int x3d_version(void) { return 100; }
+
+const char* x3d_strerror(int code)
+{
+  switch (code) {
+    case X3D_OK: return "ok";
+    case X3D_ERR_ARG: return "invalid argument";
+    case X3D_ERR_SIZE: return "array sizes do not match the context (or n < 8)";
+    case X3D_ERR_HIP: return "HIP runtime error";
+    case X3D_ERR_FFT: return "hipFFT error";
+    case X3D_ERR_ALLOC: return "device allocation failed";
+    case X3D_ERR_UNSUPPORTED: return "line length not supported by the line kernels";
+    default: return "unknown error";
+  }
+}
+
+double x3d_wtime(void)
+{
+  struct timeval tv;
+  gettimeofday(&tv, nullptr);
+  return (double)tv.tv_sec + 1e-6 * (double)tv.tv_usec;
+}
+
+int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_ctx** out)
+{
+  if (!out) return X3D_ERR_ARG;
+  *out = nullptr;
+  if (nx < 8 || ny < 8 || nz < 8) return X3D_ERR_SIZE;
+  if (!(xlx > 0) || !(yly > 0) || !(zlz > 0)) return X3D_ERR_ARG;
+  x3d_ctx* c = new x3d_ctx();
+  c->nx = nx; c->ny = ny; c->nz = nz;
+  c->xlx = xlx; c->yly = yly; c->zlz = zlz;
+  c->dx = xlx / (double)nx; c->dy = yly / (double)ny; c->dz = zlz / (double)nz;  // nXm = nX
+  int rc = X3D_OK;
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return X3D_ERR_HIP; }
+  scheme(c->sc[0], c->dx);
+  scheme(c->sc[1], c->dy);
+  scheme(c->sc[2], c->dz);
+  direction_ops(c, 0, nx);
+  direction_ops(c, 1, ny);
+  direction_ops(c, 2, nz);
+  if ((rc = line_plan(nx, false, &c->plan[0])) || (rc = line_plan(ny, true, &c->plan[1])) ||
+      (rc = line_plan(nz, true, &c->plan[2])) || (rc = upload_tables(c)) || (rc = poisson_setup(c))) {
+    x3d_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return X3D_OK;
+}
+
+int x3d_destroy(x3d_ctx* c)
+{
+  if (!c) return X3D_ERR_ARG;
+  poisson_release(c);
+  if (c->tab64) (void)hipFree(c->tab64);
+  if (c->tab32) (void)hipFree(c->tab32);
+  for (double* w : c->work)
+    if (w) (void)hipFree(w);
+  if (c->red) (void)hipFree(c->red);
+  delete c;
+  return X3D_OK;
+}
+
+int x3d_set_stream(x3d_ctx* c, void* stream)
+{
+  if (!c) return X3D_ERR_ARG;
+  c->stream = (hipStream_t)stream;
+  if (c->fft_ready) {
+    if (hipfftSetStream(c->fft_r2c, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+    if (hipfftSetStream(c->fft_c2r, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  }
+  return X3D_OK;
+}
+
+int x3d_sync(x3d_ctx* c)
+{
+  if (!c) return X3D_ERR_ARG;
+  X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_get_sizes(const x3d_ctx* c, int* nx, int* ny, int* nz)
+{
+  if (!c || !nx || !ny || !nz) return X3D_ERR_ARG;
+  *nx = c->nx; *ny = c->ny; *nz = c->nz;
+  return X3D_OK;
+}
+
+int x3d_field_alloc(x3d_ctx* c, size_t nbytes, void** p)
+{
+  if (!c || !p) return X3D_ERR_ARG;
+  X3D_CHECK_HIP(hipMalloc(p, nbytes));
+  return X3D_OK;
+}
+
+int x3d_field_free(x3d_ctx* c, void* p)
+{
+  if (!c) return X3D_ERR_ARG;
+  if (p) X3D_CHECK_HIP(hipFree(p));
+  return X3D_OK;
+}
+
+int x3d_memcpy_h2d(x3d_ctx* c, void* dst, const void* src, size_t nbytes)
+{
+  if (!c || !dst || !src) return X3D_ERR_ARG;
+  X3D_CHECK_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, c->stream));
+  X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_memcpy_d2h(x3d_ctx* c, void* dst, const void* src, size_t nbytes)
+{
+  if (!c || !dst || !src) return X3D_ERR_ARG;
+  X3D_CHECK_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, c->stream));
+  X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_operator_get(const x3d_ctx* c, int op, double* f, double* s, double* w, double* p,
+                     double* alfa, int* n, int* npaire)
+{
+  if (!c || op < 0 || op >= X3D_NOPS) return X3D_ERR_ARG;
+  const OpHost& o = c->op[op];
+  const size_t nb = sizeof(double) * o.n;
+  if (f) std::memcpy(f, o.f.data(), nb);
+  if (s) std::memcpy(s, o.s.data(), nb);
+  if (w) std::memcpy(w, o.w.data(), nb);
+  if (p) std::memcpy(p, o.p.data(), nb);
+  if (alfa) *alfa = o.alfa;
+  if (n) *n = o.n;
+  if (npaire) *npaire = o.npaire;
+  return X3D_OK;
+}
+
+int x3d_operator_set(x3d_ctx* c, int op, const double* f, const double* s, const double* w,
+                     const double* p, double alfa)
+{
+  if (!c || op < 0 || op >= X3D_NOPS || !f || !s || !w || !p) return X3D_ERR_ARG;
+  OpHost& o = c->op[op];
+  const int n = o.n;
+  o.f.assign(f, f + n); o.s.assign(s, s + n); o.w.assign(w, w + n); o.p.assign(p, p + n);
+  o.alfa = alfa;
+  X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return upload_tables(c);
+}
+
+int x3d_scheme_consts(const x3d_ctx* c, double* out)
+{
+  if (!c || !out) return X3D_ERR_ARG;
+  for (int d = 0; d < 3; ++d) {
+    const Scheme& s = c->sc[d];
+    double* o = out + 11 * d;
+    o[0] = s.alfai; o[1] = s.afi; o[2] = s.bfi; o[3] = s.alcai; o[4] = s.aci; o[5] = s.bci;
+    o[6] = s.ailcai; o[7] = s.aici; o[8] = s.bici; o[9] = s.cici; o[10] = s.dici;
+  }
+  return X3D_OK;
+}
+
+// ---- Thomas ---------------------------------------------------------------
+static int check3(const x3d_ctx* c, int nx, int ny, int nz)
+{
+  if (!c) return X3D_ERR_ARG;
+  return (nx == c->nx && ny == c->ny && nz == c->nz) ? X3D_OK : X3D_ERR_SIZE;
+}
+
+int x3d_xthomas(x3d_ctx* c, double* tt, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 0, K_NONE, op, tt, tt, true);
+}
+int x3d_ythomas(x3d_ctx* c, double* tt, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 1, K_NONE, op, tt, tt, true);
+}
+int x3d_zthomas(x3d_ctx* c, double* tt, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 2, K_NONE, op, tt, tt, true);
+}
+
+// ---- collocated derivatives ----------------------------------------------
+int x3d_derx(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 0, K_D, op, u, t, true);
+}
+int x3d_dery(x3d_ctx* c, double* t, const double* u, int op, const double* ppy, int nx, int ny, int nz)
+{
+  (void)ppy;
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 1, K_D, op, u, t, true);
+}
+int x3d_derz(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 2, K_D, op, u, t, true);
+}
+
+// ---- staggered (periodic: nXm = nX) --------------------------------------
+#define X3D_STAG(name, dir, kind, a1, a2, a3, a4, cond)                                        \
+  int name(x3d_ctx* c, double* t, const double* u, int op, int a1, int a2, int a3, int a4)    \
+  {                                                                                           \
+    if (!c) return X3D_ERR_ARG;                                                               \
+    if (!(cond)) return X3D_ERR_SIZE;                                                         \
+    return launch_line(c, dir, kind, op, u, t, true);                                         \
+  }
+
+X3D_STAG(x3d_derxvp, 0, K_DVP, nx, nxm, ny, nz, nx == c->nx && nxm == nx && ny == c->ny && nz == c->nz)
+X3D_STAG(x3d_interxvp, 0, K_IVP, nx, nxm, ny, nz, nx == c->nx && nxm == nx && ny == c->ny && nz == c->nz)
+X3D_STAG(x3d_derxpv, 0, K_DPV, nxm, nx, ny, nz, nx == c->nx && nxm == nx && ny == c->ny && nz == c->nz)
+X3D_STAG(x3d_interxpv, 0, K_IPV, nxm, nx, ny, nz, nx == c->nx && nxm == nx && ny == c->ny && nz == c->nz)
+X3D_STAG(x3d_interyvp, 1, K_IVP, nx, ny, nym, nz, nx == c->nx && ny == c->ny && nym == ny && nz == c->nz)
+X3D_STAG(x3d_interypv, 1, K_IPV, nx, nym, ny, nz, nx == c->nx && ny == c->ny && nym == ny && nz == c->nz)
+X3D_STAG(x3d_derzvp, 2, K_DVP, nx, ny, nz, nzm, nx == c->nx && ny == c->ny && nz == c->nz && nzm == nz)
+X3D_STAG(x3d_interzvp, 2, K_IVP, nx, ny, nz, nzm, nx == c->nx && ny == c->ny && nz == c->nz && nzm == nz)
+X3D_STAG(x3d_derzpv, 2, K_DPV, nx, ny, nzm, nz, nx == c->nx && ny == c->ny && nz == c->nz && nzm == nz)
+X3D_STAG(x3d_interzpv, 2, K_IPV, nx, ny, nzm, nz, nx == c->nx && ny == c->ny && nz == c->nz && nzm == nz)
+#undef X3D_STAG
+
+int x3d_deryvp(x3d_ctx* c, double* t, const double* u, int op, const double* ppyi, int nx, int ny,
+               int nym, int nz)
+{
+  (void)ppyi;
+  if (!c) return X3D_ERR_ARG;
+  if (!(nx == c->nx && ny == c->ny && nym == ny && nz == c->nz)) return X3D_ERR_SIZE;
+  return launch_line(c, 1, K_DVP, op, u, t, true);
+}
+
+int x3d_derypv(x3d_ctx* c, double* t, const double* u, int op, const double* ppy, int nx, int nym,
+               int ny, int nz)
+{
+  (void)ppy;
+  if (!c) return X3D_ERR_ARG;
+  if (!(nx == c->nx && ny == c->ny && nym == ny && nz == c->nz)) return X3D_ERR_SIZE;
+  return launch_line(c, 1, K_DPV, op, u, t, true);
+}
+
+// ---- transposes ------------------------------------------------------------
+static int tcopy(x3d_ctx* c, const void* in, void* out, size_t eb, int n1, int n2, int n3)
+{
+  if (!c || !in || !out || (eb != 4 && eb != 8 && eb != 16) || n1 <= 0 || n2 <= 0 || n3 <= 0)
+    return X3D_ERR_ARG;
+  if (in == out) return X3D_OK;
+  X3D_CHECK_HIP(hipMemcpyAsync(out, in, eb * (size_t)n1 * n2 * n3, hipMemcpyDeviceToDevice, c->stream));
+  return X3D_OK;
+}
+int x3d_transpose_x_to_y(x3d_ctx* c, const void* in, void* out, size_t eb, int n1, int n2, int n3)
+{ return tcopy(c, in, out, eb, n1, n2, n3); }
+int x3d_transpose_y_to_z(x3d_ctx* c, const void* in, void* out, size_t eb, int n1, int n2, int n3)
+{ return tcopy(c, in, out, eb, n1, n2, n3); }
+int x3d_transpose_z_to_y(x3d_ctx* c, const void* in, void* out, size_t eb, int n1, int n2, int n3)
+{ return tcopy(c, in, out, eb, n1, n2, n3); }
+int x3d_transpose_y_to_x(x3d_ctx* c, const void* in, void* out, size_t eb, int n1, int n2, int n3)
+{ return tcopy(c, in, out, eb, n1, n2, n3); }
+
+// ---- Poisson -----------------------------------------------------------------
+int x3d_poisson(x3d_ctx* c, double* rhs)
+{
+  if (!c || !rhs) return X3D_ERR_ARG;
+  return poisson_solve(c, rhs);
+}
+
+// ---- step pieces (reference-shaped composition) ------------------------------
+#define X3D_TRY(expr) do { int _rc = (expr); if (_rc) return _rc; } while (0)
+
+int x3d_momentum_rhs(x3d_ctx* c, double* dux, double* duy, double* duz, const double* ux,
+                     const double* uy, const double* uz)
+{
+  if (!c || !dux || !duy || !duz || !ux || !uy || !uz) return X3D_ERR_ARG;
+  X3D_TRY(ensure_work(c));
+  double *ta = c->work[0], *td = c->work[1], *tg = c->work[2], *t3 = c->work[3];
+  const double* u[3] = {ux, uy, uz};
+  double* du[3] = {dux, duy, duz};
+  // operator instances per component and direction, {product, field}
+  // (src/transeq.f90:106-107, 126-127, 145-146 / 192-193, 211-212, 231-232 /
+  //  278-279, 297-298, 316-317)
+  static const int opx[3][2] = {{X3D_OP_DERXP, X3D_OP_DERX}, {X3D_OP_DERX, X3D_OP_DERXP}, {X3D_OP_DERX, X3D_OP_DERXP}};
+  static const int opy[3][2] = {{X3D_OP_DERY, X3D_OP_DERYP}, {X3D_OP_DERYP, X3D_OP_DERY}, {X3D_OP_DERY, X3D_OP_DERYP}};
+  static const int opz[3][2] = {{X3D_OP_DERZ, X3D_OP_DERZP}, {X3D_OP_DERZ, X3D_OP_DERZP}, {X3D_OP_DERZP, X3D_OP_DERZ}};
+  for (int k = 0; k < 3; ++k) {
+    const double* uc = u[k];
+    X3D_TRY(ew(c, EW_MUL, ta, ux, uc, nullptr, 0));
+    X3D_TRY(launch_line(c, 0, K_D, opx[k][0], ta, td, true));
+    X3D_TRY(launch_line(c, 0, K_D, opx[k][1], uc, ta, true));
+    X3D_TRY(ew(c, EW_FMA_ADD, du[k], ux, ta, td, 0));
+    X3D_TRY(ew(c, EW_MUL, td, uc, uy, nullptr, 0));
+    X3D_TRY(launch_line(c, 1, K_D, opy[k][0], td, tg, true));
+    X3D_TRY(launch_line(c, 1, K_D, opy[k][1], uc, td, true));
+    X3D_TRY(ew(c, EW_FMA_ADD, tg, uy, td, tg, 0));
+    X3D_TRY(ew(c, EW_MUL, td, uc, uz, nullptr, 0));
+    X3D_TRY(launch_line(c, 2, K_D, opz[k][0], td, t3, true));
+    X3D_TRY(launch_line(c, 2, K_D, opz[k][1], uc, td, true));
+    X3D_TRY(ew(c, EW_FMA_ADD, t3, uz, td, t3, 0));
+    X3D_TRY(ew(c, EW_ADD, tg, tg, t3, nullptr, 0));
+    X3D_TRY(ew(c, EW_ADD, du[k], du[k], tg, nullptr, 0));
+  }
+  return X3D_OK;
+}
+
+int x3d_int_time(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, const double* dux,
+                 const double* duy, const double* duz)
+{
+  if (!c || !ux || !uy || !uz || !dux || !duy || !duz) return X3D_ERR_ARG;
+  X3D_TRY(ew(c, EW_AXPY, ux, dux, ux, nullptr, gdt));
+  X3D_TRY(ew(c, EW_AXPY, uy, duy, uy, nullptr, gdt));
+  X3D_TRY(ew(c, EW_AXPY, uz, duz, uz, nullptr, gdt));
+  return X3D_OK;
+}
+
+int x3d_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz)
+{
+  if (!c || !pp3 || !ux || !uy || !uz) return X3D_ERR_ARG;
+  X3D_TRY(ensure_work(c));
+  double *pp1 = c->work[0], *pgy1 = c->work[1], *pgz1 = c->work[2];
+  double *upi2 = c->work[3], *duydypi2 = c->work[4], *po3 = c->work[5];
+  X3D_TRY(launch_line(c, 0, K_DVP, X3D_OP_DERXVP, ux, pp1, true));
+  X3D_TRY(launch_line(c, 0, K_IVP, X3D_OP_INTXVP, uy, pgy1, true));
+  X3D_TRY(launch_line(c, 0, K_IVP, X3D_OP_INTXVP, uz, pgz1, true));
+  X3D_TRY(launch_line(c, 1, K_IVP, X3D_OP_INTYVP, pp1, upi2, true));
+  X3D_TRY(launch_line(c, 1, K_DVP, X3D_OP_DERYVP, pgy1, duydypi2, true));
+  X3D_TRY(ew(c, EW_ADD, duydypi2, duydypi2, upi2, nullptr, 0));
+  X3D_TRY(launch_line(c, 1, K_IVP, X3D_OP_INTYVP, pgz1, upi2, true));
+  X3D_TRY(launch_line(c, 2, K_IVP, X3D_OP_INTZVP, duydypi2, pp3, true));
+  X3D_TRY(launch_line(c, 2, K_DVP, X3D_OP_DERZVP, upi2, po3, true));
+  X3D_TRY(ew(c, EW_ADD, pp3, pp3, po3, nullptr, 0));
+  return X3D_OK;
+}
+
+int x3d_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3)
+{
+  if (!c || !px || !py || !pz || !pp3) return X3D_ERR_ARG;
+  X3D_TRY(ensure_work(c));
+  double *ppi3 = c->work[0], *pgz3 = c->work[1], *ppi2 = c->work[2], *pgy2 = c->work[3];
+  double* pgzi2 = c->work[4];
+  X3D_TRY(launch_line(c, 2, K_IPV, X3D_OP_INTZPV, pp3, ppi3, true));
+  X3D_TRY(launch_line(c, 2, K_DPV, X3D_OP_DERZPV, pp3, pgz3, true));
+  X3D_TRY(launch_line(c, 1, K_IPV, X3D_OP_INTYPV, ppi3, ppi2, true));
+  X3D_TRY(launch_line(c, 1, K_DPV, X3D_OP_DERYPV, ppi3, pgy2, true));
+  X3D_TRY(launch_line(c, 1, K_IPV, X3D_OP_INTYPV, pgz3, pgzi2, true));
+  X3D_TRY(launch_line(c, 0, K_DPV, X3D_OP_DERXPV, ppi2, px, true));
+  X3D_TRY(launch_line(c, 0, K_IPV, X3D_OP_INTXPV, pgy2, py, true));
+  X3D_TRY(launch_line(c, 0, K_IPV, X3D_OP_INTXPV, pgzi2, pz, true));
+  return X3D_OK;
+}
+
+int x3d_cor_vel(x3d_ctx* c, double* ux, double* uy, double* uz, const double* px, const double* py,
+                const double* pz)
+{
+  if (!c || !ux || !uy || !uz || !px || !py || !pz) return X3D_ERR_ARG;
+  X3D_TRY(ew(c, EW_SUB, ux, ux, px, nullptr, 0));
+  X3D_TRY(ew(c, EW_SUB, uy, uy, py, nullptr, 0));
+  X3D_TRY(ew(c, EW_SUB, uz, uz, pz, nullptr, 0));
+  return X3D_OK;
+}
+
+int x3d_solve_poisson(x3d_ctx* c, double* pp3, double* px, double* py, double* pz, const double* ux,
+                      const double* uy, const double* uz)
+{
+  X3D_TRY(x3d_divergence(c, pp3, ux, uy, uz));
+  X3D_TRY(x3d_poisson(c, pp3));
+  X3D_TRY(x3d_gradp(c, px, py, pz, pp3));
+  return X3D_OK;
+}
+
+int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double* dux, double* duy,
+             double* duz, double* pp3, double* px, double* py, double* pz)
+{
+  X3D_TRY(x3d_momentum_rhs(c, dux, duy, duz, ux, uy, uz));
+  X3D_TRY(x3d_int_time(c, gdt, ux, uy, uz, dux, duy, duz));
+  X3D_TRY(x3d_solve_poisson(c, pp3, px, py, pz, ux, uy, uz));
+  X3D_TRY(x3d_cor_vel(c, ux, uy, uz, px, py, pz));
+  return X3D_OK;
+}
+
+// ---- test mode and fields ----------------------------------------------------
+int x3d_test_err(x3d_ctx* c, const double* d, int which, double* err)
+{
+  if (!c || !d || !err || which < 0 || which > 2) return X3D_ERR_ARG;
+  double s, m;
+  X3D_TRY(reduce(c, d, which, &s, &m));
+  *err = std::sqrt(s / c->nx / c->ny / c->nz);
+  return X3D_OK;
+}
+
+int x3d_div_stats(x3d_ctx* c, const double* pp3, double* tmax, double* tmean)
+{
+  if (!c || !pp3 || !tmax || !tmean) return X3D_ERR_ARG;
+  double s, m;
+  X3D_TRY(reduce(c, pp3, 3, &s, &m));
+  *tmax = m;
+  *tmean = s / ((double)c->nx * c->ny * c->nz);
+  return X3D_OK;
+}
+
+int x3d_init_fields(x3d_ctx* c, int kind, uint64_t seed, double amp, double* ux, double* uy,
+                    double* uz)
+{
+  if (!c || !ux || !uy || !uz || kind < 0 || kind > 2) return X3D_ERR_ARG;
+  int hk[3 * 8 * 3];
+  double hph[3 * 8];
+  const double twopi = 2.0 * 3.14159265358979323846;
+  for (int cpt = 0; cpt < 3; ++cpt)
+    for (int m = 0; m < 8; ++m) {
+      int* kk = hk + 3 * (8 * cpt + m);
+      for (int d = 0; d < 3; ++d)
+        kk[d] = (int)std::floor(host_uniform01(seed, 16 + cpt, (uint64_t)(4 * m + d)) * 9.0) - 4;
+      if (!kk[0] && !kk[1] && !kk[2]) kk[0] = 1;
+      hph[8 * cpt + m] = twopi * host_uniform01(seed, 16 + cpt, (uint64_t)(4 * m + 3));
+    }
+  int* dk = nullptr;
+  double* dph = nullptr;
+  X3D_CHECK_HIP(hipMalloc(&dk, sizeof(hk)));
+  X3D_CHECK_HIP(hipMalloc(&dph, sizeof(hph)));
+  X3D_CHECK_HIP(hipMemcpy(dk, hk, sizeof(hk), hipMemcpyHostToDevice));
+  X3D_CHECK_HIP(hipMemcpy(dph, hph, sizeof(hph), hipMemcpyHostToDevice));
+  GridInfo g{c->nx, c->ny, c->nz, c->dx, c->dy, c->dz, c->xlx, c->yly, c->zlz};
+  hipLaunchKernelGGL(k_init_fields, dim3(4096), dim3(256), 0, c->stream, kind, seed, amp, g, dk, dph, ux, uy, uz);
+  hipError_t e = hipGetLastError();
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dk);
+  (void)hipFree(dph);
+  X3D_CHECK_HIP(e);
+  return X3D_OK;
+}
+
+}  // extern "C"

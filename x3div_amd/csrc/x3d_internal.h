@@ -1,0 +1,110 @@
+// Internal definitions of libx3dhip (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/x3dhip.h"
+
+namespace x3d {
+
+// Stencil kinds of the periodic right-hand sides.
+enum Kind : int {
+  K_D = 0,    // collocated first derivative (der?_00)
+  K_DVP = 1,  // staggered derivative, velocity -> pressure grid
+  K_DPV = 2,  // staggered derivative, pressure -> velocity grid
+  K_IVP = 3,  // mid-point interpolation, velocity -> pressure grid
+  K_IPV = 4,  // mid-point interpolation, pressure -> velocity grid
+  K_NONE = 5  // no stencil: the Thomas solve alone (in place)
+};
+
+// Scheme constants of one direction (module x3d_operator_?_data).
+struct Scheme {
+  double alfai, afi, bfi;                    // compact first derivative
+  double alcai, aci, bci;                    // staggered derivative
+  double ailcai, aici, bici, cici, dici;     // interpolation (ipinter = 3)
+};
+
+// Host copy of one x3doperator1d instance.
+struct OpHost {
+  int n = 0, npaire = 0;
+  double alfa = 0.0;
+  std::vector<double> f, s, w, p;
+};
+
+// Device view of one operator: four length-n vectors and alfa.
+template <typename T>
+struct OpDev {
+  const T* f;
+  const T* s;
+  const T* w;
+  const T* p;
+  T alfa;
+};
+
+// How the lines of one direction are tiled onto a work-group.
+struct LinePlan {
+  int n = 0;      // line length
+  int M = 0;      // points per chunk (per thread)
+  int C = 0;      // chunks per line
+  int G = 0;      // chunks of one line inside one wavefront
+  int Wc = 0;     // wavefronts cooperating on the same lines
+  int Wl = 0;     // independent line groups per work-group
+  int LW = 0;     // lines per wavefront (64 / G)
+  bool line_fast = false;  // lanes: consecutive lines (y/z) or consecutive chunks (x)
+};
+
+}  // namespace x3d
+
+struct x3d_ctx {
+  int nx = 0, ny = 0, nz = 0;
+  double xlx = 1.0, yly = 1.0, zlz = 1.0, dx = 0.0, dy = 0.0, dz = 0.0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  x3d::Scheme sc[3];
+  x3d::OpHost op[X3D_NOPS];
+  // device operator tables: for op o, f,s,w,p at tab64 + off[o] + {0,1,2,3}*n
+  double* tab64 = nullptr;
+  float* tab32 = nullptr;
+  size_t off[X3D_NOPS] = {};
+  size_t tab_len = 0;
+  x3d::LinePlan plan[3];
+  // Poisson
+  hipfftHandle fft_r2c = 0, fft_c2r = 0;
+  bool fft_ready = false;
+  double2* spec = nullptr;          // (nx/2+1) * ny * nz complex
+  double* ktab = nullptr;           // device 1-D tables, see poisson source
+  std::vector<double> ax, bx, ay, by, az, bz;
+  std::vector<double> xk2, yk2, zk2, exs, eys, ezs;
+  // step scratch fields (allocated on first use)
+  double* work[8] = {};
+  size_t work_elems = 0;
+  double* red = nullptr;            // reduction scratch (host-visible results)
+};
+
+namespace x3d {
+
+int hip_status(hipError_t e);
+int line_plan(int n, bool line_fast, LinePlan* out);
+
+// Launch one compact operator (or Thomas alone) along direction dir.
+int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
+                bool periodic_fix);
+
+int poisson_setup(x3d_ctx* c);
+void poisson_release(x3d_ctx* c);
+int poisson_solve(x3d_ctx* c, double* rhs);
+
+int ensure_work(x3d_ctx* c);
+
+}  // namespace x3d
+
+#define X3D_CHECK_HIP(expr)                        \
+  do {                                             \
+    hipError_t _e = (expr);                        \
+    if (_e != hipSuccess) return x3d::hip_status(_e); \
+  } while (0)
