@@ -176,6 +176,53 @@ def test_step_parity(grid, gdt):
     assert relerr(host(pp), rpp) < 10 * RTOL
 
 
+def test_caller_tables_take_the_table_sweeps(grid):
+    """x3d_operator_set (a caller's own x3doperator1d tables) switches every
+    operator to the reference's table sweeps and the step to the reference-shaped
+    composition; results stay at the FP64 bar."""
+    from x3div_amd import OPS, X3D
+    ctx0, orc = grid
+    ctx = X3D(orc.nx, orc.ny, orc.nz, *orc.L)
+    try:
+        for name in OPS:
+            f, s, w, p, alfa, _ = orc.op(name)
+            ctx.set_operator(name, f, s, w, p, alfa)
+        for name in ("derx", "intyvp", "derzpv"):
+            u = rand(orc, 5)
+            t = ctx.field(fill=np.nan)
+            ctx.apply(name, t, dev(u))
+            assert relerr(host(t), orc.apply(name, u)) < RTOL
+        u0 = orc.fields(1, 31, 1e-3)
+        ru, rdu, rpp, _ = orc.step(*u0, gdt=1e-3)
+        u = [dev(a) for a in u0]
+        du = [ctx.field() for _ in range(3)]
+        pp = ctx.field(); g = [ctx.field() for _ in range(3)]
+        ctx.step(*u, *du, pp, *g, gdt=1e-3)
+        for a, b in zip(u + du, ru + rdu):
+            assert relerr(host(a), b) < RTOL
+        assert relerr2(host(pp), rpp) < RTOL
+    finally:
+        ctx.close()
+
+
+def test_fused_step_matches_composition(grid):
+    """The fused step equals its four reference-shaped pieces in sequence."""
+    ctx, orc = grid
+    u0 = orc.fields(1, 8, 1e-3)
+    a = [dev(x) for x in u0]
+    b = [dev(x) for x in u0]
+    da = [ctx.field() for _ in range(3)]; db = [ctx.field() for _ in range(3)]
+    pa = ctx.field(); pb = ctx.field()
+    ga = [ctx.field() for _ in range(3)]; gb = [ctx.field() for _ in range(3)]
+    ctx.step(*a, *da, pa, *ga, gdt=2e-3)
+    ctx.calculate_transeq_rhs(*db, *b)
+    ctx.int_time(*b, *db, gdt=2e-3)
+    ctx.solve_poisson(pb, *gb, *b)
+    ctx.cor_vel(*b, *gb)
+    for x, y in zip(a + da + ga, b + db + gb):
+        assert relerr(host(x), host(y)) < 1e-13
+
+
 def test_init_fields_and_test_mode(grid):
     ctx, orc = grid
     for kind in (0, 1, 2):

@@ -57,6 +57,7 @@ static void op_init(OpHost& o, const std::vector<double>& f, const std::vector<d
   o.p.assign(n, 0.0);
   o.p[0] = -1.0;
   o.p[n - 1] = alfa;
+  o.circulant = true;
   for (int k = 1; k < n; ++k) o.p[k] = o.p[k] - o.p[k - 1] * s[k];
   o.p[n - 1] = o.p[n - 1] * w[n - 1];
   for (int k = n - 2; k >= 0; --k) o.p[k] = (o.p[k] - f[k] * o.p[k + 1]) * w[k];
@@ -363,6 +364,8 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   *out = nullptr;
   if (nx < 8 || ny < 8 || nz < 8) return X3D_ERR_SIZE;
   if (!(xlx > 0) || !(yly > 0) || !(zlz > 0)) return X3D_ERR_ARG;
+  // the line kernels address a field with 32-bit byte offsets
+  if ((unsigned long long)nx * ny * nz * sizeof(double) > 0xFFFFFFFFull) return X3D_ERR_UNSUPPORTED;
   x3d_ctx* c = new x3d_ctx();
   c->nx = nx; c->ny = ny; c->nz = nz;
   c->xlx = xlx; c->yly = yly; c->zlz = zlz;
@@ -476,6 +479,7 @@ int x3d_operator_set(x3d_ctx* c, int op, const double* f, const double* s, const
   const int n = o.n;
   o.f.assign(f, f + n); o.s.assign(s, s + n); o.w.assign(w, w + n); o.p.assign(p, p + n);
   o.alfa = alfa;
+  o.circulant = false;  // caller tables: always solved with the table sweeps
   X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
   return upload_tables(c);
 }
@@ -600,10 +604,16 @@ int x3d_poisson(x3d_ctx* c, double* rhs)
 // ---- step pieces (reference-shaped composition) ------------------------------
 #define X3D_TRY(expr) do { int _rc = (expr); if (_rc) return _rc; } while (0)
 
-int x3d_momentum_rhs(x3d_ctx* c, double* dux, double* duy, double* duz, const double* ux,
-                     const double* uy, const double* uz)
+}  // extern "C"
+
+namespace x3d {
+// Reference-shaped compositions: one line kernel per operator call and one
+// elementwise kernel per glue loop, in the reference's statement order.  Used
+// when a caller replaced operator tables (then the line kernels run the table
+// sweeps); otherwise the fused passes of x3d_fused.hip run.
+static int ref_momentum_rhs(x3d_ctx* c, double* dux, double* duy, double* duz, const double* ux,
+                            const double* uy, const double* uz)
 {
-  if (!c || !dux || !duy || !duz || !ux || !uy || !uz) return X3D_ERR_ARG;
   X3D_TRY(ensure_work(c));
   double *ta = c->work[0], *td = c->work[1], *tg = c->work[2], *t3 = c->work[3];
   const double* u[3] = {ux, uy, uz};
@@ -634,19 +644,8 @@ int x3d_momentum_rhs(x3d_ctx* c, double* dux, double* duy, double* duz, const do
   return X3D_OK;
 }
 
-int x3d_int_time(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, const double* dux,
-                 const double* duy, const double* duz)
+static int ref_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz)
 {
-  if (!c || !ux || !uy || !uz || !dux || !duy || !duz) return X3D_ERR_ARG;
-  X3D_TRY(ew(c, EW_AXPY, ux, dux, ux, nullptr, gdt));
-  X3D_TRY(ew(c, EW_AXPY, uy, duy, uy, nullptr, gdt));
-  X3D_TRY(ew(c, EW_AXPY, uz, duz, uz, nullptr, gdt));
-  return X3D_OK;
-}
-
-int x3d_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz)
-{
-  if (!c || !pp3 || !ux || !uy || !uz) return X3D_ERR_ARG;
   X3D_TRY(ensure_work(c));
   double *pp1 = c->work[0], *pgy1 = c->work[1], *pgz1 = c->work[2];
   double *upi2 = c->work[3], *duydypi2 = c->work[4], *po3 = c->work[5];
@@ -663,9 +662,8 @@ int x3d_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, 
   return X3D_OK;
 }
 
-int x3d_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3)
+static int ref_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3)
 {
-  if (!c || !px || !py || !pz || !pp3) return X3D_ERR_ARG;
   X3D_TRY(ensure_work(c));
   double *ppi3 = c->work[0], *pgz3 = c->work[1], *ppi2 = c->work[2], *pgy2 = c->work[3];
   double* pgzi2 = c->work[4];
@@ -678,6 +676,43 @@ int x3d_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3)
   X3D_TRY(launch_line(c, 0, K_IPV, X3D_OP_INTXPV, pgy2, py, true));
   X3D_TRY(launch_line(c, 0, K_IPV, X3D_OP_INTXPV, pgzi2, pz, true));
   return X3D_OK;
+}
+}  // namespace x3d
+
+extern "C" {
+
+int x3d_momentum_rhs(x3d_ctx* c, double* dux, double* duy, double* duz, const double* ux,
+                     const double* uy, const double* uz)
+{
+  if (!c || !dux || !duy || !duz || !ux || !uy || !uz) return X3D_ERR_ARG;
+  if (!fused_ok(c)) return ref_momentum_rhs(c, dux, duy, duz, ux, uy, uz);
+  // the fused passes do not write u when euler = 0
+  return fused_momentum(c, 0.0, 0, const_cast<double*>(ux), const_cast<double*>(uy),
+                        const_cast<double*>(uz), dux, duy, duz);
+}
+
+int x3d_int_time(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, const double* dux,
+                 const double* duy, const double* duz)
+{
+  if (!c || !ux || !uy || !uz || !dux || !duy || !duz) return X3D_ERR_ARG;
+  X3D_TRY(ew(c, EW_AXPY, ux, dux, ux, nullptr, gdt));
+  X3D_TRY(ew(c, EW_AXPY, uy, duy, uy, nullptr, gdt));
+  X3D_TRY(ew(c, EW_AXPY, uz, duz, uz, nullptr, gdt));
+  return X3D_OK;
+}
+
+int x3d_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz)
+{
+  if (!c || !pp3 || !ux || !uy || !uz) return X3D_ERR_ARG;
+  if (!fused_ok(c)) return ref_divergence(c, pp3, ux, uy, uz);
+  return fused_divergence(c, pp3, ux, uy, uz);
+}
+
+int x3d_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3)
+{
+  if (!c || !px || !py || !pz || !pp3) return X3D_ERR_ARG;
+  if (!fused_ok(c)) return ref_gradp(c, px, py, pz, pp3);
+  return fused_gradp(c, px, py, pz, pp3, nullptr, nullptr, nullptr, 0);
 }
 
 int x3d_cor_vel(x3d_ctx* c, double* ux, double* uy, double* uz, const double* px, const double* py,
@@ -702,10 +737,20 @@ int x3d_solve_poisson(x3d_ctx* c, double* pp3, double* px, double* py, double* p
 int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double* dux, double* duy,
              double* duz, double* pp3, double* px, double* py, double* pz)
 {
-  X3D_TRY(x3d_momentum_rhs(c, dux, duy, duz, ux, uy, uz));
-  X3D_TRY(x3d_int_time(c, gdt, ux, uy, uz, dux, duy, duz));
-  X3D_TRY(x3d_solve_poisson(c, pp3, px, py, pz, ux, uy, uz));
-  X3D_TRY(x3d_cor_vel(c, ux, uy, uz, px, py, pz));
+  if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  if (!fused_ok(c)) {
+    X3D_TRY(x3d_momentum_rhs(c, dux, duy, duz, ux, uy, uz));
+    X3D_TRY(x3d_int_time(c, gdt, ux, uy, uz, dux, duy, duz));
+    X3D_TRY(x3d_solve_poisson(c, pp3, px, py, pz, ux, uy, uz));
+    X3D_TRY(x3d_cor_vel(c, ux, uy, uz, px, py, pz));
+    return X3D_OK;
+  }
+  // transeq with the Euler update folded into its z pass, divergence, Poisson,
+  // gradp with cor_vel folded into its x pass
+  X3D_TRY(fused_momentum(c, gdt, 1, ux, uy, uz, dux, duy, duz));
+  X3D_TRY(fused_divergence(c, pp3, ux, uy, uz));
+  X3D_TRY(poisson_solve(c, pp3));
+  X3D_TRY(fused_gradp(c, px, py, pz, pp3, ux, uy, uz, 1));
   return X3D_OK;
 }
 

@@ -1,244 +1,64 @@
-// Periodic compact operators on gfx950: stencil right-hand side, Thomas
-// forward/backward sweeps and the Sherman-Morrison periodic correction in one
-// pass over each line (one global read of the input window, one write).
-//
-// The reference runs each line's sweeps sequentially (src/thomas.f90:61-84
-// and the y/z twins), so one line per thread needs the whole line's
-// intermediate on chip.  Here a line of n points is cut into C chunks of M
-// consecutive points, one chunk per thread, and both sweeps run as chunked
-// first-order linear recurrences:
-//   forward  t_j = r_j - s_j t_{j-1}        (carry-less local pass, then
-//   backward x_j = (t_j - f_j x_{j+1}) w_j   a scan of the chunk maps
-//                                           x -> A x + B and a fix-up).
-// The scan composes the chunks of a line inside a wavefront with cross-lane
-// shuffles and across wavefronts through LDS.  The coefficients are the
-// operator's own f, s, w and periodic vectors (x3doperator1d), so the
-// arithmetic is the reference's Thomas algorithm, reassociated.
-//
-// Lane layouts: x-lines are contiguous, so consecutive lanes take
-// consecutive chunks of one line; y/z-lines are strided, so consecutive
-// lanes take 16 consecutive lines (128 B rows) and the chunks of a line are
-// spread over lane groups and wavefronts.
-#include "x3d_internal.h"
+// Single-operator periodic line kernels: stencil right-hand side, periodic
+// tridiagonal solve and store, one pass over each line (one read of the
+// input window, one write).  Drop-ins for derx_00/dery_00/derz_00, the twelve
+// staggered routines and xthomas_0/ythomas_0/zthomas_0.
+#include <cmath>
+
+#include "x3d_line.h"
 
 namespace x3d {
 
-template <int KIND>
-struct Halo { static constexpr int L = 0, R = 0; };
-template <> struct Halo<K_D> { static constexpr int L = 2, R = 2; };
-template <> struct Halo<K_DVP> { static constexpr int L = 1, R = 2; };
-template <> struct Halo<K_DPV> { static constexpr int L = 2, R = 1; };
-template <> struct Halo<K_IVP> { static constexpr int L = 3, R = 4; };
-template <> struct Halo<K_IPV> { static constexpr int L = 4, R = 3; };
+double circ_gamma(double alfa) { return 0.5 * (1.0 + std::sqrt(1.0 - 4.0 * alfa * alfa)); }
 
 template <typename T>
-struct Coef { T a, b, c, d; };
-
-// Right-hand side at window index q (window holds u_{j0-HL} ...).
-// D    derx_00      src/x3d_derive.f90:134-137
-// DVP  derxvp       src/x3d_staggered.f90:48-51
-// DPV  derxpv       src/x3d_staggered.f90:246-249
-// IVP  interxvp     src/x3d_staggered.f90:143-148
-// IPV  interxpv     src/x3d_staggered.f90:326-331
-// (the wrapped boundary rows are the same formulas with indices mod n)
-template <int KIND, typename T>
-__device__ __forceinline__ T rhs_at(const T* win, int q, const Coef<T>& k)
+Circ<T> make_circ(double alfa, const LinePlan& P)
 {
-  if constexpr (KIND == K_D)
-    return k.a * (win[q + 1] - win[q - 1]) + k.b * (win[q + 2] - win[q - 2]);
-  else if constexpr (KIND == K_DVP)
-    return k.a * (win[q + 1] - win[q]) + k.b * (win[q + 2] - win[q - 1]);
-  else if constexpr (KIND == K_DPV)
-    return k.a * (win[q] - win[q - 1]) + k.b * (win[q + 1] - win[q - 2]);
-  else if constexpr (KIND == K_IVP)
-    return k.a * (win[q + 1] + win[q]) + k.b * (win[q + 2] + win[q - 1]) +
-           k.c * (win[q + 3] + win[q - 2]) + k.d * (win[q + 4] + win[q - 3]);
-  else if constexpr (KIND == K_IPV)
-    return k.a * (win[q] + win[q - 1]) + k.b * (win[q + 1] + win[q - 2]) +
-           k.c * (win[q + 2] + win[q - 3]) + k.d * (win[q + 3] + win[q - 4]);
-  else
-    return win[q];
-}
-
-struct Geom {
-  int dir;        // 0 x, 1 y, 2 z
-  int nx;
-  long long nxy;  // nx * ny
-  int n;          // line length
-  int nlines;
-};
-
-__device__ __forceinline__ long long line_base(const Geom& g, int line)
-{
-  if (g.dir == 0) return (long long)line * g.nx;
-  if (g.dir == 1) {
-    const int k = line / g.nx;
-    return (long long)(line - k * g.nx) + (long long)k * g.nxy;
+  const double gam = circ_gamma(alfa);
+  const double beta = -alfa / gam;
+  Circ<T> c{};
+  c.beta = (T)beta;
+  double pw = 1.0;
+  for (int k = 0; k <= 16; ++k) {
+    c.pw[k] = (T)(k <= P.M ? pw : 0.0);
+    pw *= beta;
   }
-  return line;
+  double bM = std::pow(beta, P.M);
+  for (int k = 0; k < 7; ++k) {
+    c.pwM2[k] = (T)bM;
+    bM = bM * bM;
+  }
+  c.pwMG = (T)std::pow(beta, (double)P.M * P.G);
+  c.close = (T)(1.0 / (1.0 - std::pow(beta, (double)P.n)));
+  return c;
 }
+template Circ<double> make_circ<double>(double, const LinePlan&);
+template Circ<float> make_circ<float>(double, const LinePlan&);
 
-__device__ __forceinline__ long long point_stride(const Geom& g)
+Coef<double> stencil_coef(const Scheme& s, int kind)
 {
-  return g.dir == 0 ? 1 : (g.dir == 1 ? (long long)g.nx : g.nxy);
+  switch (kind) {
+    case K_D: return {s.afi, s.bfi, 0.0, 0.0};
+    case K_DVP: case K_DPV: return {s.aci, s.bci, 0.0, 0.0};
+    case K_IVP: case K_IPV: return {s.aici, s.bici, s.cici, s.dici};
+    default: return {0.0, 0.0, 0.0, 0.0};
+  }
 }
 
-struct Tiling {
-  int G, Wc, Wl, LW, line_fast, fix;
-};
-
-template <typename T, int KIND, int M>
-__global__ __launch_bounds__(1024) void k_line(const T* in, T* out, OpDev<T> op, Coef<T> cf,
+template <typename T, int KIND, int M, bool CIRC, bool LF>
+__global__ __launch_bounds__(1024) void k_line(const T* in, T* out, OpDev<T> op, Coef<T> cf, Circ<T> cc,
                                                Geom geo, Tiling tl)
 {
-  constexpr int HL = Halo<KIND>::L, HR = Halo<KIND>::R, NW = M + HL + HR;
+  constexpr int HL = Halo<KIND>::L, HR = Halo<KIND>::R;
   extern __shared__ unsigned char smem_raw[];
-  T* sm = reinterpret_cast<T*>(smem_raw);
-
-  const int G = tl.G, Wc = tl.Wc, LW = tl.LW;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wl = wave / Wc, wc = wave - wl * Wc;
-  int lineLocal, g;
-  if (tl.line_fast) { lineLocal = lane % LW; g = lane / LW; }
-  else { lineLocal = lane / G; g = lane - lineLocal * G; }
-  const int LPW = tl.Wl * LW;                 // lines per work-group
-  const int ls = wl * LW + lineLocal;         // line slot in the work-group
-  const int line = blockIdx.x * LPW + ls;
-  const bool active = line < geo.nlines;
-  const int n = geo.n;
-  const int C = G * Wc;
-  const int c = wc * G + g;                   // chunk index along the line
-  const int j0 = c * M;
-  // lane holding chunk gg of this thread's line inside the wavefront
-  auto lane_of = [&](int gg) { return tl.line_fast ? lineLocal + gg * LW : lineLocal * G + gg; };
-
-  T* sA = sm;                    // forward aggregates  [Wc][LPW]
-  T* sB = sA + Wc * LPW;
-  T* rA = sB + Wc * LPW;         // backward aggregates [Wc][LPW]
-  T* rB = rA + Wc * LPW;
-  T* s0 = rB + Wc * LPW;         // x_1 and x_n of each line [LPW]
-  T* sN = s0 + LPW;
-
-  long long base = 0, ps = 1;
-  if (active) { base = line_base(geo, line); ps = point_stride(geo); }
-
-  T win[NW];
-#pragma unroll
-  for (int q = 0; q < NW; ++q) {
-    int j = j0 - HL + q;
-    j = j < 0 ? j + n : (j >= n ? j - n : j);
-    win[q] = active ? in[base + (long long)j * ps] : T(0);
-  }
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  const LineThread t = line_thread<T>(geo, tl, M);
+  T win[M + HL + HR];
+  load_window<T, M, HL, HR, LF>(in, t, geo.n, win);
   T v[M];
-#pragma unroll
-  for (int jj = 0; jj < M; ++jj) v[jj] = rhs_at<KIND>(win, jj + HL, cf);
-
-  // ---- forward sweep t_j = r_j - s_j t_{j-1} (src/thomas.f90:73-75) ----
-  T A = T(1), B = T(0);
-  {
-    T prev = T(0);
-#pragma unroll
-    for (int jj = 0; jj < M; ++jj) {
-      const T s = op.s[j0 + jj];
-      v[jj] = v[jj] - prev * s;
-      prev = v[jj];
-      A = A * (-s);
-    }
-    B = prev;
-  }
-  for (int d = 1; d < G; d <<= 1) {
-    const int src = lane_of(g >= d ? g - d : g);
-    const T Ao = __shfl(A, src), Bo = __shfl(B, src);
-    if (g >= d) { B = A * Bo + B; A = A * Ao; }
-  }
-  T carry;
-  {
-    const int src = lane_of(g > 0 ? g - 1 : 0);
-    const T Ap = __shfl(A, src), Bp = __shfl(B, src);
-    const T Ae = g > 0 ? Ap : T(1), Be = g > 0 ? Bp : T(0);
-    if (Wc > 1) {
-      if (g == G - 1) { sA[wc * LPW + ls] = A; sB[wc * LPW + ls] = B; }
-      __syncthreads();
-      T cw = T(0);
-      for (int w2 = 0; w2 < wc; ++w2) cw = sA[w2 * LPW + ls] * cw + sB[w2 * LPW + ls];
-      carry = Ae * cw + Be;
-    } else {
-      carry = Be;
-    }
-  }
-  {
-    T P = T(1);
-#pragma unroll
-    for (int jj = 0; jj < M; ++jj) {
-      P = P * (-op.s[j0 + jj]);
-      v[jj] = v[jj] + P * carry;
-    }
-  }
-
-  // ---- backward sweep x_j = (t_j - f_j x_{j+1}) w_j (src/thomas.f90:76-79) ----
-  A = T(1);
-  {
-    T nxt = T(0);
-#pragma unroll
-    for (int jj = M - 1; jj >= 0; --jj) {
-      const T f = op.f[j0 + jj], w = op.w[j0 + jj];
-      v[jj] = (v[jj] - f * nxt) * w;
-      nxt = v[jj];
-      A = A * (-(f * w));
-    }
-    B = nxt;
-  }
-  for (int d = 1; d < G; d <<= 1) {
-    const int src = lane_of(g + d < G ? g + d : g);
-    const T Ao = __shfl(A, src), Bo = __shfl(B, src);
-    if (g + d < G) { B = A * Bo + B; A = A * Ao; }
-  }
-  {
-    const int src = lane_of(g < G - 1 ? g + 1 : g);
-    const T Ap = __shfl(A, src), Bp = __shfl(B, src);
-    const T Ae = g < G - 1 ? Ap : T(1), Be = g < G - 1 ? Bp : T(0);
-    if (Wc > 1) {
-      if (g == 0) { rA[wc * LPW + ls] = A; rB[wc * LPW + ls] = B; }
-      __syncthreads();
-      T cw = T(0);
-      for (int w2 = Wc - 1; w2 > wc; --w2) cw = rA[w2 * LPW + ls] * cw + rB[w2 * LPW + ls];
-      carry = Ae * cw + Be;
-    } else {
-      carry = Be;
-    }
-  }
-  {
-    T Q = T(1);
-#pragma unroll
-    for (int jj = M - 1; jj >= 0; --jj) {
-      const T f = op.f[j0 + jj], w = op.w[j0 + jj];
-      Q = Q * (-(f * w));
-      v[jj] = v[jj] + Q * carry;
-    }
-  }
-
-  // ---- periodic correction (src/thomas.f90:49-55) ----
-  if (tl.fix) {
-    T x0, xn;
-    if (Wc > 1) {
-      if (c == 0) s0[ls] = v[0];
-      if (c == C - 1) sN[ls] = v[M - 1];
-      __syncthreads();
-      x0 = s0[ls];
-      xn = sN[ls];
-    } else {
-      x0 = __shfl(v[0], lane_of(0));
-      xn = __shfl(v[M - 1], lane_of(G - 1));
-    }
-    const T ss = (x0 - op.alfa * xn) / (T(1) + op.p[0] - op.alfa * op.p[n - 1]);
-#pragma unroll
-    for (int jj = 0; jj < M; ++jj) v[jj] = v[jj] - ss * op.p[j0 + jj];
-  }
-
-  if (active) {
-#pragma unroll
-    for (int jj = 0; jj < M; ++jj) out[base + (long long)(j0 + jj) * ps] = v[jj];
-  }
+  rhs_chunk<KIND, T, M, HL>(win, cf, v);
+  if constexpr (CIRC) solve_circ<T, M, LF>(v, t, cc, lds);
+  else solve_table<T, M, LF>(v, t, op, lds, geo.n, tl.fix != 0);
+  store_chunk<T, M>(out, t, v);
 }
 
 // Points per chunk the kernels are instantiated for.
@@ -271,12 +91,31 @@ int line_plan(int n, bool line_fast, LinePlan* P)
   return X3D_OK;
 }
 
-template <typename T, int KIND>
-static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStream_t st,
-                              const T* in, T* out, OpDev<T> op, Coef<T> cf, Geom geo, Tiling tl)
+void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, dim3* block, size_t* smem,
+                 size_t tbytes)
 {
-#define X3D_CASE(MM) \
-  case MM: hipLaunchKernelGGL((k_line<T, KIND, MM>), grid, block, smem, st, in, out, op, cf, geo, tl); break;
+  const LinePlan& P = c->plan[dir];
+  geo->dir = dir; geo->nx = c->nx; geo->nxy = (long long)c->nx * c->ny; geo->n = P.n;
+  geo->nlines = dir == 0 ? c->ny * c->nz : dir == 1 ? c->nx * c->nz : c->nx * c->ny;
+  tl->G = P.G; tl->Wc = P.Wc; tl->Wl = P.Wl; tl->LW = P.LW; tl->line_fast = P.line_fast ? 1 : 0;
+  tl->fix = 1;
+  const int lpw = P.Wl * P.LW;
+  *grid = dim3((geo->nlines + lpw - 1) / lpw);
+  *block = dim3(64 * P.Wc * P.Wl);
+  *smem = tbytes * (size_t)line_lds_words(P.Wc, lpw);
+}
+
+template <typename T, int KIND, bool CIRC>
+static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStream_t st, const T* in, T* out,
+                              OpDev<T> op, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl)
+{
+#define X3D_CASE(MM)                                                                                   \
+  case MM:                                                                                             \
+    if (tl.line_fast)                                                                                  \
+      hipLaunchKernelGGL((k_line<T, KIND, MM, CIRC, true>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_line<T, KIND, MM, CIRC, false>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
+    break;
   switch (M) {
     X3D_CASE(1) X3D_CASE(2) X3D_CASE(3) X3D_CASE(4) X3D_CASE(5) X3D_CASE(6)
     X3D_CASE(8) X3D_CASE(12) X3D_CASE(16)
@@ -286,8 +125,22 @@ static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStre
   return hipGetLastError();
 }
 
-int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
-                bool periodic_fix)
+template <bool CIRC>
+static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem, hipStream_t st,
+                             const double* in, double* out, OpDev<double> od, Coef<double> cf,
+                             Circ<double> cc, Geom geo, Tiling tl)
+{
+  switch (kind) {
+    case K_D: return launch_kind<double, K_D, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_DVP: return launch_kind<double, K_DVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_DPV: return launch_kind<double, K_DPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_IVP: return launch_kind<double, K_IVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_IPV: return launch_kind<double, K_IPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out, bool periodic_fix)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS || !in || !out) return X3D_ERR_ARG;
   const LinePlan& P = c->plan[dir];
@@ -296,34 +149,29 @@ int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double*
   if (P.M == 0) return X3D_ERR_UNSUPPORTED;
 
   Geom geo;
-  geo.dir = dir; geo.nx = c->nx; geo.nxy = (long long)c->nx * c->ny; geo.n = n;
-  geo.nlines = dir == 0 ? c->ny * c->nz : dir == 1 ? c->nx * c->nz : c->nx * c->ny;
   Tiling tl;
-  tl.G = P.G; tl.Wc = P.Wc; tl.Wl = P.Wl; tl.LW = P.LW; tl.line_fast = P.line_fast ? 1 : 0;
+  dim3 grid, block;
+  size_t smem;
+  launch_dims(c, dir, &geo, &tl, &grid, &block, &smem, sizeof(double));
   tl.fix = periodic_fix ? 1 : 0;
-  const int lpw = P.Wl * P.LW;
-  dim3 grid((geo.nlines + lpw - 1) / lpw), block(64 * P.Wc * P.Wl);
-  const size_t smem = P.Wc > 1 ? sizeof(double) * (size_t)(4 * P.Wc * lpw + 2 * lpw) : 0;
 
-  const double* t = c->tab64 + c->off[op];
-  OpDev<double> od{t, t + n, t + 2 * n, t + 3 * n, c->op[op].alfa};
-  const Scheme& s = c->sc[dir];
-  Coef<double> cf{0, 0, 0, 0};
-  switch (kind) {
-    case K_D: cf = {s.afi, s.bfi, 0, 0}; break;
-    case K_DVP: case K_DPV: cf = {s.aci, s.bci, 0, 0}; break;
-    case K_IVP: case K_IPV: cf = {s.aici, s.bici, s.cici, s.dici}; break;
-    default: break;
-  }
+  const double* tb = c->tab64 + c->off[op];
+  OpDev<double> od{tb, tb + n, tb + 2 * n, tb + 3 * n, c->op[op].alfa};
+  Coef<double> cf = stencil_coef(c->sc[dir], kind);
+  Circ<double> cc{};
   hipError_t e;
-  switch (kind) {
-    case K_D: e = launch_kind<double, K_D>(P.M, grid, block, smem, c->stream, in, out, od, cf, geo, tl); break;
-    case K_DVP: e = launch_kind<double, K_DVP>(P.M, grid, block, smem, c->stream, in, out, od, cf, geo, tl); break;
-    case K_DPV: e = launch_kind<double, K_DPV>(P.M, grid, block, smem, c->stream, in, out, od, cf, geo, tl); break;
-    case K_IVP: e = launch_kind<double, K_IVP>(P.M, grid, block, smem, c->stream, in, out, od, cf, geo, tl); break;
-    case K_IPV: e = launch_kind<double, K_IPV>(P.M, grid, block, smem, c->stream, in, out, od, cf, geo, tl); break;
-    case K_NONE: e = launch_kind<double, K_NONE>(P.M, grid, block, smem, c->stream, in, out, od, cf, geo, tl); break;
-    default: return X3D_ERR_ARG;
+  // the circulant solver applies when the operator carries the library's own
+  // periodic tables and the full periodic solve is asked for
+  const bool circ = kind != K_NONE && periodic_fix && c->op[op].circulant;
+  if (circ) {
+    cc = make_circ<double>(c->op[op].alfa, P);
+    const double ig = 1.0 / circ_gamma(c->op[op].alfa);
+    cf = {cf.a * ig, cf.b * ig, cf.c * ig, cf.d * ig};
+    e = launch_any<true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
+  } else if (kind == K_NONE) {
+    e = launch_kind<double, K_NONE, false>(P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
+  } else {
+    e = launch_any<false>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   }
   return e == hipSuccess ? X3D_OK : hip_status(e);
 }

@@ -33,6 +33,7 @@ struct Scheme {
 struct OpHost {
   int n = 0, npaire = 0;
   double alfa = 0.0;
+  bool circulant = false;  // tables are the library's periodic [alfa, 1, alfa] ones
   std::vector<double> f, s, w, p;
 };
 
@@ -95,11 +96,24 @@ int line_plan(int n, bool line_fast, LinePlan* out);
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
                 bool periodic_fix);
 
+struct Geom;
+struct Tiling;
+void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, dim3* block, size_t* smem,
+                 size_t tbytes);
+
 int poisson_setup(x3d_ctx* c);
 void poisson_release(x3d_ctx* c);
 int poisson_solve(x3d_ctx* c, double* rhs);
 
 int ensure_work(x3d_ctx* c);
+
+// Fused step passes (all operators must carry the library's periodic tables).
+bool fused_ok(const x3d_ctx* c);
+int fused_momentum(x3d_ctx* c, double gdt, int euler, double* ux, double* uy, double* uz, double* dux,
+                   double* duy, double* duz);
+int fused_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz);
+int fused_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3, double* ux, double* uy,
+                double* uz, int corr);
 
 }  // namespace x3d
 

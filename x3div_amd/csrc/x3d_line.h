@@ -1,0 +1,413 @@
+// Device building blocks of the periodic line kernels (stencil right-hand
+// sides, thread tiling and the two line solvers).
+//
+// A line of n points is cut into C chunks of M consecutive points, one chunk
+// per thread.  Lane layouts: x-lines are contiguous, so consecutive lanes take
+// consecutive chunks of one line (tl.line_fast = 0); y/z-lines are strided,
+// so consecutive lanes take consecutive lines (16 lines = 128 B rows) and the
+// chunks of a line are spread over 4 lane groups and over wavefronts.
+//
+// Solvers (both solve the reference's periodic tridiagonal system: Thomas
+// sweeps on the matrix prepared by src/x3d_operator_1d.f90:301-323, then the
+// Sherman-Morrison correction of src/thomas.f90:49-55):
+//  * solve_table: the reference's sweeps with the operator's own f, s, w and
+//    periodic vectors, reassociated into chunked first-order recurrences
+//    (used for the Thomas entry points and for caller-supplied tables);
+//  * solve_circ: the same periodic system [alfa, 1, alfa] solved with the
+//    limit factorisation of its Thomas sweeps, gamma (1 - beta L)(1 - beta U)
+//    with gamma = (1 + sqrt(1 - 4 alfa^2)) / 2 and beta = -alfa / gamma, and
+//    the cyclic wrap closed exactly by a geometric series instead of the
+//    rank-one correction.  It needs no per-point coefficient loads.
+#pragma once
+
+#include "x3d_internal.h"
+
+namespace x3d {
+
+template <int KIND>
+struct Halo { static constexpr int L = 0, R = 0; };
+template <> struct Halo<K_D> { static constexpr int L = 2, R = 2; };
+template <> struct Halo<K_DVP> { static constexpr int L = 1, R = 2; };
+template <> struct Halo<K_DPV> { static constexpr int L = 2, R = 1; };
+template <> struct Halo<K_IVP> { static constexpr int L = 3, R = 4; };
+template <> struct Halo<K_IPV> { static constexpr int L = 4, R = 3; };
+
+template <typename T>
+struct Coef { T a, b, c, d; };
+
+// Right-hand side at window index q.
+// D    derx_00      src/x3d_derive.f90:134-137
+// DVP  derxvp       src/x3d_staggered.f90:48-51
+// DPV  derxpv       src/x3d_staggered.f90:246-249
+// IVP  interxvp     src/x3d_staggered.f90:143-148
+// IPV  interxpv     src/x3d_staggered.f90:326-331
+// (the wrapped boundary rows are the same formulas with indices mod n)
+template <int KIND, typename T>
+__device__ __forceinline__ T rhs_at(const T* win, int q, const Coef<T>& k)
+{
+  if constexpr (KIND == K_D)
+    return k.a * (win[q + 1] - win[q - 1]) + k.b * (win[q + 2] - win[q - 2]);
+  else if constexpr (KIND == K_DVP)
+    return k.a * (win[q + 1] - win[q]) + k.b * (win[q + 2] - win[q - 1]);
+  else if constexpr (KIND == K_DPV)
+    return k.a * (win[q] - win[q - 1]) + k.b * (win[q + 1] - win[q - 2]);
+  else if constexpr (KIND == K_IVP)
+    return k.a * (win[q + 1] + win[q]) + k.b * (win[q + 2] + win[q - 1]) +
+           k.c * (win[q + 3] + win[q - 2]) + k.d * (win[q + 4] + win[q - 3]);
+  else if constexpr (KIND == K_IPV)
+    return k.a * (win[q] + win[q - 1]) + k.b * (win[q + 1] + win[q - 2]) +
+           k.c * (win[q + 2] + win[q - 3]) + k.d * (win[q + 3] + win[q - 4]);
+  else
+    return win[q];
+}
+
+struct Geom {
+  int dir;        // 0 x, 1 y, 2 z
+  int nx;
+  long long nxy;  // nx * ny
+  int n;          // line length
+  int nlines;
+};
+
+struct Tiling {
+  int G, Wc, Wl, LW, line_fast, fix;
+};
+
+// Constants of the circulant solver for one alfa and one tiling.
+template <typename T>
+struct Circ {
+  T beta;
+  T close;       // 1 / (1 - beta^n)
+  T pwMG;        // beta^(M G)
+  T pwM2[7];     // beta^(M 2^k)
+  T pw[17];      // beta^k, k = 0..M
+};
+
+struct LineThread {
+  int lineLocal, g, wc, ls, LPW, line, c, C, j0, G, Wc, LW, lf;
+  bool active;
+  uint32_t boff;  // byte offset of the line's first point
+  uint32_t bps;   // byte stride between consecutive points of the line
+  __device__ __forceinline__ int lane_of(int gg) const { return lf ? lineLocal + gg * LW : lineLocal * G + gg; }
+  __device__ __forceinline__ uint32_t at(int j) const { return boff + (uint32_t)j * bps; }
+};
+
+// Element load/store at a 32-bit byte offset from a uniform base (fields are
+// below 4 GiB; the host checks), which maps onto saddr-form global accesses.
+template <typename T>
+__device__ __forceinline__ T ld(const T* base, uint32_t boff)
+{
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st(T* base, uint32_t boff, T v)
+{
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// DIRC >= 0 fixes the direction at compile time; otherwise geo.dir selects it.
+template <typename T, int DIRC = -1>
+__device__ __forceinline__ LineThread line_thread(const Geom& geo, const Tiling& tl, int M)
+{
+  LineThread t;
+  t.G = tl.G; t.Wc = tl.Wc; t.LW = tl.LW; t.lf = tl.line_fast;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wl = wave / tl.Wc;
+  t.wc = wave - wl * tl.Wc;
+  if (tl.line_fast) { t.lineLocal = lane % tl.LW; t.g = lane / tl.LW; }
+  else { t.lineLocal = lane / tl.G; t.g = lane - t.lineLocal * tl.G; }
+  t.LPW = tl.Wl * tl.LW;
+  t.ls = wl * tl.LW + t.lineLocal;
+  t.line = blockIdx.x * t.LPW + t.ls;
+  t.active = t.line < geo.nlines;
+  t.C = tl.G * tl.Wc;
+  t.c = t.wc * tl.G + t.g;
+  t.j0 = t.c * M;
+  // lanes past the last line read the last line (results are not stored), so
+  // every load is unconditional
+  const int ln = t.active ? t.line : geo.nlines - 1;
+  // Line base and point stride are formed without branches: a three-way
+  // if/else on the (uniform) direction here was compiled by the ROCm 7.2
+  // AMDGPU backend so that the z path kept the y stride.
+  const int dir = DIRC >= 0 ? DIRC : geo.dir;
+  const uint32_t nx = (uint32_t)geo.nx, nxy = (uint32_t)geo.nxy, l = (uint32_t)ln;
+  const uint32_t k = l / nx;
+  const uint32_t ex = l * nx, ey = (l - k * nx) + k * nxy, ez = l;
+  const uint32_t e0 = dir == 0 ? ex : (dir == 1 ? ey : ez);
+  const uint32_t eps = dir == 0 ? 1u : (dir == 1 ? nx : nxy);
+  t.boff = e0 * (uint32_t)sizeof(T);
+  t.bps = eps * (uint32_t)sizeof(T);
+  return t;
+}
+
+// u at points j0-HL .. j0+M+HR-1 of this thread's line (wrapped mod n).
+// x-lines (consecutive lanes hold consecutive chunks): the M chunk points are
+// loaded from one address with immediate offsets and the halo comes from the
+// neighbouring lanes; strided y/z-lines load every point.
+template <typename T, int M, int HL, int HR, bool LF>
+__device__ __forceinline__ void load_window(const T* __restrict__ in, const LineThread& t, int n,
+                                            T (&win)[M + HL + HR])
+{
+  if constexpr (!LF) {
+    const T* p = reinterpret_cast<const T*>(reinterpret_cast<const char*>(in) + t.at(t.j0));
+#pragma unroll
+    for (int k = 0; k < M; ++k) win[HL + k] = p[k];
+#pragma unroll
+    for (int h = 1; h <= HL; ++h) {
+      const int d = (h + M - 1) / M;           // chunks to the left
+      const int idx = M * d - h;               // index inside that chunk
+      const int src = t.lane_of((t.g - d % t.G + t.G) % t.G);
+      win[HL - h] = __shfl(win[HL + idx], src);
+    }
+#pragma unroll
+    for (int h = 1; h <= HR; ++h) {
+      const int d = (h + M - 1) / M;           // chunks to the right
+      const int idx = h - 1 - M * (d - 1);
+      const int src = t.lane_of((t.g + d) % t.G);
+      win[HL + M - 1 + h] = __shfl(win[HL + idx], src);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < M + HL + HR; ++q) {
+      int j = t.j0 - HL + q;
+      j = j < 0 ? j + n : (j >= n ? j - n : j);
+      win[q] = ld(in, t.at(j));
+    }
+  }
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void load_chunk(const T* __restrict__ in, const LineThread& t, T (&v)[M])
+{
+#pragma unroll
+  for (int k = 0; k < M; ++k) v[k] = ld(in, t.at(t.j0 + k));
+}
+
+template <typename T, int M>
+__device__ __forceinline__ void store_chunk(T* out, const LineThread& t, const T (&v)[M])
+{
+  if (t.active) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) st(out, t.at(t.j0 + k), v[k]);
+  }
+}
+
+// Right-hand side of KIND for the M chunk points from a window whose halo is
+// HLW points on the left (HLW >= Halo<KIND>::L).
+template <int KIND, typename T, int M, int HLW, int NW>
+__device__ __forceinline__ void rhs_chunk(const T (&win)[NW], const Coef<T>& cf, T (&v)[M])
+{
+#pragma unroll
+  for (int k = 0; k < M; ++k) v[k] = rhs_at<KIND>(win, k + HLW, cf);
+}
+
+// LDS words (of T) the solvers use per work-group.
+__host__ __device__ inline int line_lds_words(int Wc, int lpw) { return Wc > 1 ? 4 * Wc * lpw + 2 * lpw : 0; }
+
+// beta^(M c) from the binary powers.
+template <typename T>
+__device__ __forceinline__ T pow_Mc(const Circ<T>& cc, int c)
+{
+  T r = T(1);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) r = ((c >> k) & 1) ? r * cc.pwM2[k] : r;
+  return r;
+}
+
+// ---- circulant solver ------------------------------------------------------
+template <typename T, int M, bool LF>
+__device__ __forceinline__ void solve_circ(T (&v)[M], const LineThread& t, const Circ<T>& cc, T* lds)
+{
+  const int G = t.G, g = t.g;
+  T* agf = lds;                          // [Wc][LPW] forward wave aggregates
+  T* agb = lds + t.Wc * t.LPW;           // [Wc][LPW] backward wave aggregates
+  // forward y_j = q_j + beta y_{j-1}
+  T prev = T(0);
+#pragma unroll
+  for (int k = 0; k < M; ++k) { prev = v[k] + cc.beta * prev; v[k] = prev; }
+  T E = prev;
+  {
+    int s = 0;
+    for (int d = 1; d < G; d <<= 1, ++s) {
+      const T o = __shfl(E, t.lane_of(g >= d ? g - d : g));
+      if (g >= d) E = E + cc.pwM2[s] * o;
+    }
+  }
+  T Eex = __shfl(E, t.lane_of(g > 0 ? g - 1 : 0));
+  if (g == 0) Eex = T(0);
+  T cw = T(0), total;
+  if (LF && t.Wc > 1) {
+    if (g == G - 1) agf[t.wc * t.LPW + t.ls] = E;
+    __syncthreads();
+    T acc = T(0);
+    for (int w = 0; w < t.Wc; ++w) {
+      if (w == t.wc) cw = acc;
+      acc = cc.pwMG * acc + agf[w * t.LPW + t.ls];
+    }
+    total = acc;
+  } else {
+    total = __shfl(E, t.lane_of(G - 1));
+  }
+  const T ylast = total * cc.close;
+  const T carry = pow_Mc(cc, t.c) * ylast + (Eex + pow_Mc(cc, g) * cw);
+#pragma unroll
+  for (int k = 0; k < M; ++k) v[k] = v[k] + cc.pw[k + 1] * carry;
+
+  // backward x_j = y_j + beta x_{j+1}
+  T nxt = T(0);
+#pragma unroll
+  for (int k = M - 1; k >= 0; --k) { nxt = v[k] + cc.beta * nxt; v[k] = nxt; }
+  T F = nxt;
+  {
+    int s = 0;
+    for (int d = 1; d < G; d <<= 1, ++s) {
+      const T o = __shfl(F, t.lane_of(g + d < G ? g + d : g));
+      if (g + d < G) F = F + cc.pwM2[s] * o;
+    }
+  }
+  T Fex = __shfl(F, t.lane_of(g < G - 1 ? g + 1 : g));
+  if (g == G - 1) Fex = T(0);
+  T cw2 = T(0), total2;
+  if (LF && t.Wc > 1) {
+    if (g == 0) agb[t.wc * t.LPW + t.ls] = F;
+    __syncthreads();
+    T acc = T(0);
+    for (int w = t.Wc - 1; w >= 0; --w) {
+      if (w == t.wc) cw2 = acc;
+      acc = cc.pwMG * acc + agb[w * t.LPW + t.ls];
+    }
+    total2 = acc;
+  } else {
+    total2 = __shfl(F, t.lane_of(0));
+  }
+  const T x0 = total2 * cc.close;
+  const T carry2 = (Fex + pow_Mc(cc, G - 1 - g) * cw2) + pow_Mc(cc, t.C - 1 - t.c) * x0;
+#pragma unroll
+  for (int k = 0; k < M; ++k) v[k] = v[k] + cc.pw[M - k] * carry2;
+}
+
+// ---- table solver (the reference's sweeps with the operator's vectors) -------
+template <typename T, int M, bool LF>
+__device__ __forceinline__ void solve_table(T (&v)[M], const LineThread& t, const OpDev<T>& op, T* lds,
+                                            int n, bool fix)
+{
+  const int G = t.G, g = t.g, Wc = LF ? t.Wc : 1, LPW = t.LPW, ls = t.ls, j0 = t.j0;
+  T* sA = lds;
+  T* sB = sA + Wc * LPW;
+  T* rA = sB + Wc * LPW;
+  T* rB = rA + Wc * LPW;
+  T* s0 = rB + Wc * LPW;
+  T* sN = s0 + LPW;
+
+  // forward t_j = r_j - s_j t_{j-1} (src/thomas.f90:73-75)
+  T A = T(1), B = T(0);
+  {
+    T prev = T(0);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const T s = op.s[j0 + k];
+      v[k] = v[k] - prev * s;
+      prev = v[k];
+      A = A * (-s);
+    }
+    B = prev;
+  }
+  for (int d = 1; d < G; d <<= 1) {
+    const int src = t.lane_of(g >= d ? g - d : g);
+    const T Ao = __shfl(A, src), Bo = __shfl(B, src);
+    if (g >= d) { B = A * Bo + B; A = A * Ao; }
+  }
+  T carry;
+  {
+    const int src = t.lane_of(g > 0 ? g - 1 : 0);
+    const T Ap = __shfl(A, src), Bp = __shfl(B, src);
+    const T Ae = g > 0 ? Ap : T(1), Be = g > 0 ? Bp : T(0);
+    if (Wc > 1) {
+      if (g == G - 1) { sA[t.wc * LPW + ls] = A; sB[t.wc * LPW + ls] = B; }
+      __syncthreads();
+      T cw = T(0);
+      for (int w2 = 0; w2 < t.wc; ++w2) cw = sA[w2 * LPW + ls] * cw + sB[w2 * LPW + ls];
+      carry = Ae * cw + Be;
+    } else {
+      carry = Be;
+    }
+  }
+  {
+    T P = T(1);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      P = P * (-op.s[j0 + k]);
+      v[k] = v[k] + P * carry;
+    }
+  }
+
+  // backward x_j = (t_j - f_j x_{j+1}) w_j (src/thomas.f90:76-79)
+  A = T(1);
+  {
+    T nxt = T(0);
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+      const T f = op.f[j0 + k], w = op.w[j0 + k];
+      v[k] = (v[k] - f * nxt) * w;
+      nxt = v[k];
+      A = A * (-(f * w));
+    }
+    B = nxt;
+  }
+  for (int d = 1; d < G; d <<= 1) {
+    const int src = t.lane_of(g + d < G ? g + d : g);
+    const T Ao = __shfl(A, src), Bo = __shfl(B, src);
+    if (g + d < G) { B = A * Bo + B; A = A * Ao; }
+  }
+  {
+    const int src = t.lane_of(g < G - 1 ? g + 1 : g);
+    const T Ap = __shfl(A, src), Bp = __shfl(B, src);
+    const T Ae = g < G - 1 ? Ap : T(1), Be = g < G - 1 ? Bp : T(0);
+    if (Wc > 1) {
+      if (g == 0) { rA[t.wc * LPW + ls] = A; rB[t.wc * LPW + ls] = B; }
+      __syncthreads();
+      T cw = T(0);
+      for (int w2 = Wc - 1; w2 > t.wc; --w2) cw = rA[w2 * LPW + ls] * cw + rB[w2 * LPW + ls];
+      carry = Ae * cw + Be;
+    } else {
+      carry = Be;
+    }
+  }
+  {
+    T Q = T(1);
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+      const T f = op.f[j0 + k], w = op.w[j0 + k];
+      Q = Q * (-(f * w));
+      v[k] = v[k] + Q * carry;
+    }
+  }
+
+  // periodic correction (src/thomas.f90:49-55)
+  if (fix) {
+    T x0, xn;
+    if (Wc > 1) {
+      if (t.c == 0) s0[ls] = v[0];
+      if (t.c == t.C - 1) sN[ls] = v[M - 1];
+      __syncthreads();
+      x0 = s0[ls];
+      xn = sN[ls];
+    } else {
+      x0 = __shfl(v[0], t.lane_of(0));
+      xn = __shfl(v[M - 1], t.lane_of(G - 1));
+    }
+    const T ss = (x0 - op.alfa * xn) / (T(1) + op.p[0] - op.alfa * op.p[n - 1]);
+#pragma unroll
+    for (int k = 0; k < M; ++k) v[k] = v[k] - ss * op.p[j0 + k];
+  }
+}
+
+// Host: circulant constants for alfa on a line plan.
+template <typename T>
+Circ<T> make_circ(double alfa, const LinePlan& P);
+// Host: stencil coefficients of a kind in a direction; scaled by 1/gamma for
+// the circulant solver when gamma_scale is true.
+Coef<double> stencil_coef(const Scheme& s, int kind);
+double circ_gamma(double alfa);
+
+}  // namespace x3d
