@@ -4,6 +4,7 @@
 #include <sys/time.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "x3d_internal.h"
@@ -378,8 +379,14 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   direction_ops(c, 0, nx);
   direction_ops(c, 1, ny);
   direction_ops(c, 2, nz);
+  // tiling variant of the fused y/z passes (X3D_FUSED_TILING=0|1, default 0)
+  const char* tv = std::getenv("X3D_FUSED_TILING");
+  const int variant = tv ? std::atoi(tv) : 0;
   if ((rc = line_plan(nx, false, &c->plan[0])) || (rc = line_plan(ny, true, &c->plan[1])) ||
-      (rc = line_plan(nz, true, &c->plan[2])) || (rc = upload_tables(c)) || (rc = poisson_setup(c))) {
+      (rc = line_plan(nz, true, &c->plan[2])) || (rc = line_plan_fused(nx, false, &c->fplan[0], variant)) ||
+      (rc = line_plan_fused(ny, true, &c->fplan[1], variant)) ||
+      (rc = line_plan_fused(nz, true, &c->fplan[2], variant)) ||
+      (rc = upload_tables(c)) || (rc = poisson_setup(c))) {
     x3d_destroy(c);
     return rc;
   }

@@ -91,10 +91,42 @@ int line_plan(int n, bool line_fast, LinePlan* P)
   return X3D_OK;
 }
 
-void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, dim3* block, size_t* smem,
-                 size_t tbytes)
+// Fused passes hold several line solves per thread, so their y/z tiling caps a
+// work-group at 8 wavefronts (up to 256 VGPRs each) and takes longer chunks:
+// the largest C <= 32 chunks per line with 4 chunk groups x 16 lines per
+// wavefront, else 64 chunks with 8 groups x 8 lines.
+int line_plan_fused(int n, bool line_fast, LinePlan* P, int variant)
 {
-  const LinePlan& P = c->plan[dir];
+  if (!line_fast) return line_plan(n, false, P);
+  if (n < 8) return X3D_ERR_SIZE;
+  auto okM = [](int m) { for (int mm : kMs) if (mm == m) return true; return false; };
+  LinePlan p;
+  p.n = n; p.line_fast = true;
+  // variant 1: 64 short chunks per line, 8 lines x 8 chunk groups per wavefront
+  if (variant == 1 && n % 64 == 0 && okM(n / 64)) {
+    p.C = 64; p.M = n / 64; p.G = 8; p.Wc = 8; p.Wl = 1; p.LW = 8;
+    *P = p;
+    return X3D_OK;
+  }
+  for (int c = 32; c >= 1; c >>= 1) {
+    if (n % c || !okM(n / c)) continue;
+    p.C = c; p.M = n / c; p.G = c < 4 ? c : 4; p.Wc = c / p.G;
+    p.Wl = p.Wc >= 4 ? 1 : 4 / p.Wc; p.LW = 64 / p.G;
+    *P = p;
+    return X3D_OK;
+  }
+  if (n % 64 == 0 && okM(n / 64)) {
+    p.C = 64; p.M = n / 64; p.G = 8; p.Wc = 8; p.Wl = 1; p.LW = 8;
+    *P = p;
+    return X3D_OK;
+  }
+  return X3D_ERR_UNSUPPORTED;
+}
+
+void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, dim3* block, size_t* smem,
+                 size_t tbytes, bool fused)
+{
+  const LinePlan& P = fused ? c->fplan[dir] : c->plan[dir];
   geo->dir = dir; geo->nx = c->nx; geo->nxy = (long long)c->nx * c->ny; geo->n = P.n;
   geo->nlines = dir == 0 ? c->ny * c->nz : dir == 1 ? c->nx * c->nz : c->nx * c->ny;
   tl->G = P.G; tl->Wc = P.Wc; tl->Wl = P.Wl; tl->LW = P.LW; tl->line_fast = P.line_fast ? 1 : 0;

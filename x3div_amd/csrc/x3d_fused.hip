@@ -17,7 +17,7 @@ namespace x3d {
 // Z: du_c = du_c + (tmp_c + r_c); if euler: u_c = gdt du_c + u_c.
 // (src/transeq.f90:100-115, 120-134, 139-153, 158-177 per component)
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(1024) void k_transeq(T* ux, T* uy, T* uz, T* dux, T* duy, T* duz, T* tx,
+__global__ __launch_bounds__(512) void k_transeq(T* ux, T* uy, T* uz, T* dux, T* duy, T* duz, T* tx,
                                                   T* ty, T* tz, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl,
                                                   T gdt, int euler)
 {
@@ -27,9 +27,11 @@ __global__ __launch_bounds__(1024) void k_transeq(T* ux, T* uy, T* uz, T* dux, T
   const LineThread t = line_thread<T, DIR>(geo, tl, M);
   const int n = geo.n;
   T* const Ud = DIR == 0 ? ux : (DIR == 1 ? uy : uz);
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
-    // uniform pointer selects (no private pointer arrays)
+    // components one after another: no loads hoisted across them
+    __builtin_amdgcn_sched_barrier(0);
+    // pointers straight from the kernel arguments (c is a constant here)
     T* const Uc = c == 0 ? ux : (c == 1 ? uy : uz);
     T* const DUc = c == 0 ? dux : (c == 1 ? duy : duz);
     T* const TMc = c == 0 ? tx : (c == 1 ? ty : tz);
@@ -48,8 +50,12 @@ __global__ __launch_bounds__(1024) void k_transeq(T* ux, T* uy, T* uz, T* dux, T
                 cf.b * (wd[q + 2] * wc[q + 2] - wd[q - 2] * wc[q - 2]);
       }
     }
+    // keep the two line solves apart: interleaving them doubles the live state
+    __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v2, t, cc, lds);
+    __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v1, t, cc, lds);
+    __builtin_amdgcn_sched_barrier(0);
     // u_d at the chunk points again (a cache hit) rather than kept live
 #pragma unroll
     for (int k = 0; k < M; ++k) v1[k] = v1[k] + ld(Ud, t.at(t.j0 + k)) * v2[k];
@@ -76,7 +82,7 @@ __global__ __launch_bounds__(1024) void k_transeq(T* ux, T* uy, T* uz, T* dux, T
 // Y: o1 = Dvp(b) + Ivp(a), o2 = Ivp(c)             a, b, c = pp1, pgy1, pgz1
 // Z: o1 = Ivp(a) + Dvp(b)                          a, b = duydypi2, upi2
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(1024) void k_div(const T* a, const T* b, const T* c, T* o1, T* o2, T* o3,
+__global__ __launch_bounds__(512) void k_div(const T* a, const T* b, const T* c, T* o1, T* o2, T* o3,
                                               Coef<T> cfd, Circ<T> ccd, Coef<T> cfi, Circ<T> cci, Geom geo,
                                               Tiling tl)
 {
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(1024) void k_div(const T* a, const T* b, const T* c
 // X: o1 = Dpv(a), o2 = Ipv(b), o3 = Ipv(c)          a, b, c = ppi2, pgy2, pgzi2
 //    if corr: u_k = u_k - o_k (src/navier.f90:84-86)
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(1024) void k_grad(const T* a, const T* b, const T* c, T* o1, T* o2, T* o3,
+__global__ __launch_bounds__(512) void k_grad(const T* a, const T* b, const T* c, T* o1, T* o2, T* o3,
                                                T* ux, T* uy, T* uz, Coef<T> cfd, Circ<T> ccd, Coef<T> cfi,
                                                Circ<T> cci, Geom geo, Tiling tl, int corr)
 {
@@ -228,8 +234,8 @@ struct DirSet {
 static DirSet dirset(const x3d_ctx* c, int dir)
 {
   DirSet s;
-  launch_dims(c, dir, &s.geo, &s.tl, &s.grid, &s.block, &s.smem, sizeof(double));
-  const LinePlan& P = c->plan[dir];
+  launch_dims(c, dir, &s.geo, &s.tl, &s.grid, &s.block, &s.smem, sizeof(double), true);
+  const LinePlan& P = c->fplan[dir];
   s.M = P.M;
   const Scheme& sc = c->sc[dir];
   auto scaled = [](Coef<double> k, double alfa) {

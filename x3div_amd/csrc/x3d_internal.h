@@ -73,7 +73,8 @@ struct x3d_ctx {
   float* tab32 = nullptr;
   size_t off[X3D_NOPS] = {};
   size_t tab_len = 0;
-  x3d::LinePlan plan[3];
+  x3d::LinePlan plan[3];   // single-operator kernels
+  x3d::LinePlan fplan[3];  // fused step passes (at most 512 threads per work-group)
   // Poisson
   hipfftHandle fft_r2c = 0, fft_c2r = 0;
   bool fft_ready = false;
@@ -91,6 +92,7 @@ namespace x3d {
 
 int hip_status(hipError_t e);
 int line_plan(int n, bool line_fast, LinePlan* out);
+int line_plan_fused(int n, bool line_fast, LinePlan* out, int variant);
 
 // Launch one compact operator (or Thomas alone) along direction dir.
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
@@ -99,7 +101,7 @@ int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double*
 struct Geom;
 struct Tiling;
 void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, dim3* block, size_t* smem,
-                 size_t tbytes);
+                 size_t tbytes, bool fused = false);
 
 int poisson_setup(x3d_ctx* c);
 void poisson_release(x3d_ctx* c);

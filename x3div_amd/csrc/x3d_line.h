@@ -92,17 +92,32 @@ struct LineThread {
   __device__ __forceinline__ uint32_t at(int j) const { return boff + (uint32_t)j * bps; }
 };
 
-// Element load/store at a 32-bit byte offset from a uniform base (fields are
-// below 4 GiB; the host checks), which maps onto saddr-form global accesses.
+// Element load/store at a 32-bit byte offset from a uniform field base (fields
+// are below 4 GiB; the host checks).  They are buffer instructions: the base
+// lives in an SGPR resource descriptor and each access costs one VGPR offset,
+// with constant offsets folded into the instruction.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* p)
+{
+  // raw buffer, stride 0, full 32-bit range; dword 3 as for gfx9 data buffers
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xFFFFFFFFu, 0x00020000);
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 template <typename T>
 __device__ __forceinline__ T ld(const T* base, uint32_t boff)
 {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(field_rsrc(base), boff, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(field_rsrc(base), boff, 0, 0));
 }
 template <typename T>
 __device__ __forceinline__ void st(T* base, uint32_t boff, T v)
 {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), field_rsrc(base), boff, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), field_rsrc(base), boff, 0, 0);
 }
 
 // DIRC >= 0 fixes the direction at compile time; otherwise geo.dir selects it.
@@ -149,9 +164,9 @@ __device__ __forceinline__ void load_window(const T* __restrict__ in, const Line
                                             T (&win)[M + HL + HR])
 {
   if constexpr (!LF) {
-    const T* p = reinterpret_cast<const T*>(reinterpret_cast<const char*>(in) + t.at(t.j0));
+    const uint32_t o = t.at(t.j0);
 #pragma unroll
-    for (int k = 0; k < M; ++k) win[HL + k] = p[k];
+    for (int k = 0; k < M; ++k) win[HL + k] = ld(in, o + (uint32_t)(k * sizeof(T)));
 #pragma unroll
     for (int h = 1; h <= HL; ++h) {
       const int d = (h + M - 1) / M;           // chunks to the left
