@@ -39,39 +39,48 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch a plain device copy of one field (counter calibration) "
-                         "and the derivative kernels, for rocprofv3 --pmc runs")
+                         "and whole steps, for rocprofv3 --pmc runs")
     return ap.parse_args()
 
 
-def profile_kernels(n: int, reps: int = 5):
+def profile_kernels(n: int, reps: int = 3):
     """Launches for hardware-counter runs: a device-to-device copy of one n^3 FP64
-    field (known bytes: 8 n^3 read + 8 n^3 written) and derx/dery/derz."""
+    field (known bytes: 8 n^3 read + 8 n^3 written, the counter calibration)
+    and whole steps, whose line passes are then read per kernel."""
     import numpy as np
     import torch
 
     from x3div_amd import X3D
     ctx = X3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi)
-    u, v, w, t = (ctx.field() for _ in range(4))
-    ctx.init_fields(u, v, w, kind=1, seed=1234, amp=1e-3)
+    f = [ctx.field() for _ in range(10)]
+    ctx.init_fields(f[0], f[1], f[2], kind=1, seed=1234, amp=1e-3)
+    t = ctx.field()
     with torch.cuda.stream(ctx.stream):
         for _ in range(reps):
-            t.copy_(u)
+            t.copy_(f[0])
     for _ in range(reps):
-        ctx.derx(t, u)
-        ctx.dery(t, u)
-        ctx.derz(t, u)
+        ctx.step(*f, gdt=0.0)
     torch.cuda.synchronize()
     ctx.close()
-    print(f"profile-kernels: {n}^3, {reps} copies of {8 * n ** 3} B and {reps} x derx/dery/derz", flush=True)
+    print(f"profile-kernels: {n}^3, {reps} copies of {8 * n ** 3} B and {reps} steps", flush=True)
 
 
-def traffic_from_profile(n: int):
-    """HBM bytes per launch of the dominant kernel from the committed PMC summary
+# the kernel template each line pass of the step launches
+KERNEL_OF_PASS = {
+    "transeq_x": "k_transeq<double, M, 0>", "transeq_y": "k_transeq<double, M, 1>",
+    "transeq_z": "k_transeq<double, M, 2>", "divergence_x": "k_div<double, M, 0>",
+    "divergence_y": "k_div<double, M, 1>", "divergence_z": "k_div<double, M, 2>",
+    "gradp_z": "k_grad<double, M, 2>", "gradp_y": "k_grad<double, M, 1>", "gradp_x": "k_grad<double, M, 0>",
+}
+
+
+def traffic_from_profile(pass_name: str, n: int):
+    """HBM bytes per launch of one pass's kernel from the committed PMC summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if one exists for this grid."""
-    path = os.path.join(ROOT, "profiles", f"k_line_derivative_{n}cubed_pmc.txt")
+    path = os.path.join(ROOT, "profiles", f"step_passes_{n}cubed_pmc.txt")
     if not os.path.exists(path):
         return None
-    m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    m = re.search(rf"^hbm_bytes_per_launch\s+{pass_name}:\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
 
 
@@ -136,37 +145,39 @@ def main():
     gpu_ms_step = ev0.elapsed_time(ev1) / args.steps
     value = n ** 3 / (ms_step / 1e3) / 1e6
 
-    # dominant kernel: the compact line kernel (collocated derivative), x/y/z
-    # launches in the same 1:1:1 mix as in the step
-    t = ctx.field()
-    for fn in (ctx.derx, ctx.dery, ctx.derz):
-        fn(t, ux)
-    torch.cuda.synchronize()
-    k0 = torch.cuda.Event(enable_timing=True)
-    k1 = torch.cuda.Event(enable_timing=True)
-    k0.record(stream)
+    # per-pass times of the same step, HIP events on the library's stream; the
+    # dominant kernel is the slowest line pass (the Poisson pass is several
+    # hipFFT kernels and is reported beside it)
+    acc = {}
     for _ in range(args.kernel_reps):
-        ctx.derx(t, ux)
-        ctx.dery(t, ux)
-        ctx.derz(t, ux)
-    k1.record(stream)
-    torch.cuda.synchronize()
-    kms = k0.elapsed_time(k1) / (3 * args.kernel_reps)
-    alg_bytes = 2 * 8 * n ** 3
+        for name, (ms, bpp) in ctx.step_timed(ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt=0.0).items():
+            acc.setdefault(name, [0.0, bpp])[0] += ms
+    passes = {}
+    for name, (ms, bpp) in acc.items():
+        ms /= args.kernel_reps
+        passes[name] = {"ms": round(ms, 4), "bytes_per_point": bpp,
+                        "GBps": round(bpp * n ** 3 / (ms / 1e3) / 1e9, 1)}
+    dom = max((k for k in passes if k != "poisson"), key=lambda k: passes[k]["ms"])
+    kms = passes[dom]["ms"]
+    alg_bytes = int(passes[dom]["bytes_per_point"] * n ** 3)
     achieved = alg_bytes / (kms / 1e3) / 1e9
+    step_bytes = sum(p["bytes_per_point"] for p in passes.values()) * n ** 3
     roofline = {
         "bound": "hbm",
-        "kernel": "k_line<double, K_D> (derx/dery/derz)",
+        "kernel": f"{dom} ({KERNEL_OF_PASS[dom]})",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profile(n),
+        "traffic": traffic_from_profile(dom, n),
         "avg_launch_ms": round(kms, 4),
         "algorithmic_bytes_per_launch": alg_bytes,
+        "step_algorithmic_bytes": int(step_bytes),
+        "step_frac": round(step_bytes / (gpu_ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "passes": passes,
     }
     ctx.close()
-    del f, t
+    del f
     torch.cuda.empty_cache()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_n)

@@ -170,6 +170,20 @@ int x3d_step(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
              double *dux, double *duy, double *duz, double *pp3,
              double *px, double *py, double *pz);
 
+/* Per-pass timing of one step, the role of the reference's NVTX ranges
+ * around each call of the step loop (src/xcompact3d.f90:79-93).  Runs
+ * x3d_step with a HIP event after each of its X3D_NPASS passes on the
+ * context's stream, waits for it, and writes each pass's milliseconds to
+ * pass_ms[X3D_NPASS].  Pass i is named by x3d_pass_name(i); its algorithmic
+ * HBM bytes per grid point (every field it reads or writes touched once) are
+ * x3d_pass_bytes_per_point(i). */
+#define X3D_NPASS 10
+int x3d_step_timed(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
+                   double *dux, double *duy, double *duz, double *pp3,
+                   double *px, double *py, double *pz, float *pass_ms);
+const char *x3d_pass_name(int pass);
+double x3d_pass_bytes_per_point(int pass);
+
 /* ---- test mode and fields --------------------------------------------- */
 /* test_du/test_dv/test_dw RMS error vs the analytic derivative of vel()
  * (src/mom.f90:59-243); which = 0,1,2.  Synchronises. */

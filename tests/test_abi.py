@@ -57,3 +57,25 @@ def test_create_rejects_bad_sizes_without_gpu_work():
     lib.x3d_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_void_p)]
     assert lib.x3d_create(4, 16, 16, 1.0, 1.0, 1.0, ctypes.byref(h)) == -2
     assert lib.x3d_create(16, 16, 16, -1.0, 1.0, 1.0, ctypes.byref(h)) == -1
+
+
+def fortran_bindings():
+    names = set()
+    fdir = os.path.join(ROOT, "x3div_amd", "fortran")
+    for f in os.listdir(fdir):
+        if f.endswith(".f90"):
+            names |= set(re.findall(r'bind\(C,\s*name="(x3d_[a-z0-9_]+)"\)', open(os.path.join(fdir, f)).read()))
+    return names
+
+
+def test_fortran_binds_only_declared_symbols():
+    """Every bind(C) name of the Fortran modules is a header entry point, and the
+    hot-path entries the reference's modules call are all bound."""
+    bound = fortran_bindings()
+    assert bound <= set(declared_symbols()), bound - set(declared_symbols())
+    hot = {"x3d_xthomas", "x3d_ythomas", "x3d_zthomas", "x3d_derx", "x3d_dery", "x3d_derz",
+           "x3d_poisson", "x3d_momentum_rhs", "x3d_int_time", "x3d_divergence", "x3d_gradp",
+           "x3d_cor_vel", "x3d_solve_poisson"}
+    hot |= {f"x3d_{k}{d}{s}" for k in ("der", "inter") for d in "xyz" for s in ("vp", "pv")}
+    hot |= {f"x3d_transpose_{a}" for a in ("x_to_y", "y_to_z", "z_to_y", "y_to_x")}
+    assert hot <= bound, hot - bound

@@ -11,6 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libx3dhip.so")
 
 X3D_OK = 0
+X3D_NPASS = 10
 ERRORS = {
     -1: "X3D_ERR_ARG", -2: "X3D_ERR_SIZE", -3: "X3D_ERR_HIP", -4: "X3D_ERR_FFT",
     -5: "X3D_ERR_ALLOC", -6: "X3D_ERR_UNSUPPORTED",
@@ -67,6 +68,9 @@ SIGNATURES = {
     "x3d_cor_vel": (_i, [_vp] + [_dp] * 6),
     "x3d_solve_poisson": (_i, [_vp] + [_dp] * 7),
     "x3d_step": (_i, [_vp, _d] + [_dp] * 10),
+    "x3d_step_timed": (_i, [_vp, _d] + [_dp] * 10 + [C.POINTER(C.c_float)]),
+    "x3d_pass_name": (C.c_char_p, [_i]),
+    "x3d_pass_bytes_per_point": (_d, [_i]),
     "x3d_test_err": (_i, [_vp, _dp, _i, _hp]),
     "x3d_div_stats": (_i, [_vp, _dp, _hp, _hp]),
     "x3d_init_fields": (_i, [_vp, _i, C.c_uint64, _d, _dp, _dp, _dp]),

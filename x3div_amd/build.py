@@ -36,6 +36,38 @@ def _compile(src: str) -> str:
     return obj
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = os.path.join(ROCM, "bin", "amdflang")
+# module order: each file uses modules of the files before it
+FSOURCES = ["x3d_hip.f90", "x3d_hip_step.f90", "xcompact3d.f90"]
+FDRIVER = os.path.join(HERE, "xcompact3d_hip")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the Fortran modules and the xcompact3d driver against libx3dhip."""
+    fbuild = os.path.join(BUILD, "fortran")
+    os.makedirs(fbuild, exist_ok=True)
+    objs = []
+    for src in FSOURCES:
+        obj = os.path.join(fbuild, os.path.splitext(src)[0] + ".o")
+        cmd = [FC, "-O2", "-c", os.path.join(FORTRAN, src), "-module-dir", fbuild,
+               "-I", fbuild, "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {src}:\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    tmp = FDRIVER + ".tmp"
+    cmd = [FC, *objs, "-o", tmp, f"-L{HERE}", "-lx3dhip", "-Wl,-rpath,$ORIGIN",
+           f"-Wl,-rpath,{ROCM}/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, FDRIVER)
+    if verbose:
+        print(f"built {FDRIVER}")
+    return FDRIVER
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -54,4 +86,5 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_fortran(verbose=True)
     sys.exit(0)

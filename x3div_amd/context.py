@@ -255,6 +255,16 @@ class X3D:
         check("x3d_step", self.lib.x3d_step(
             self._h, float(gdt), *(self._f(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
 
+    def step_timed(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """One step with a HIP event after each pass; waits for it and returns
+        {pass name: (milliseconds, algorithmic HBM bytes per grid point)}."""
+        from ._lib import X3D_NPASS
+        ms = (C.c_float * X3D_NPASS)()
+        check("x3d_step_timed", self.lib.x3d_step_timed(
+            self._h, float(gdt), *(self._f(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz)), ms))
+        return {self.lib.x3d_pass_name(i).decode(): (ms[i], self.lib.x3d_pass_bytes_per_point(i))
+                for i in range(X3D_NPASS)}
+
     # ---- test mode / fields (src/mom.f90, src/navier.f90:213-231) --------------------
     def test_err(self, d, which: int) -> float:
         e = C.c_double()

@@ -403,6 +403,8 @@ int x3d_destroy(x3d_ctx* c)
   for (double* w : c->work)
     if (w) (void)hipFree(w);
   if (c->red) (void)hipFree(c->red);
+  for (hipEvent_t e : c->pass_ev)
+    if (e) (void)hipEventDestroy(e);
   delete c;
   return X3D_OK;
 }
@@ -757,7 +759,44 @@ int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double*
   X3D_TRY(fused_momentum(c, gdt, 1, ux, uy, uz, dux, duy, duz));
   X3D_TRY(fused_divergence(c, pp3, ux, uy, uz));
   X3D_TRY(poisson_solve(c, pp3));
+  X3D_TRY(mark_pass(c, 7));
   X3D_TRY(fused_gradp(c, px, py, pz, pp3, ux, uy, uz, 1));
+  return X3D_OK;
+}
+
+namespace {
+// Pass names and algorithmic HBM bytes per grid point (FP64: each field read or
+// written once per pass).  transeq Z reads u, du, tmp and writes du, u; gradp X
+// reads three gradients and u and writes p gradients and u.  The Poisson figure
+// is one r2c read/write, one in-place spectral update and one c2r read/write.
+struct PassInfo {
+  const char* name;
+  int fields;
+};
+constexpr PassInfo kPass[X3D_NPASS] = {
+    {"transeq_x", 6}, {"transeq_y", 6}, {"transeq_z", 15}, {"divergence_x", 6}, {"divergence_y", 5},
+    {"divergence_z", 3}, {"poisson", 6}, {"gradp_z", 3}, {"gradp_y", 5}, {"gradp_x", 12}};
+}  // namespace
+
+const char* x3d_pass_name(int i) { return (i >= 0 && i < X3D_NPASS) ? kPass[i].name : nullptr; }
+
+double x3d_pass_bytes_per_point(int i) { return (i >= 0 && i < X3D_NPASS) ? 8.0 * kPass[i].fields : 0.0; }
+
+int x3d_step_timed(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double* dux, double* duy,
+                   double* duz, double* pp3, double* px, double* py, double* pz, float* pass_ms)
+{
+  if (!c || !pass_ms) return X3D_ERR_ARG;
+  if (!fused_ok(c)) return X3D_ERR_UNSUPPORTED;
+  for (int i = 0; i <= X3D_NPASS; ++i)
+    if (!c->pass_ev[i]) X3D_CHECK_HIP(hipEventCreate(&c->pass_ev[i]));
+  X3D_CHECK_HIP(hipEventRecord(c->pass_ev[0], c->stream));
+  c->timing = true;
+  const int rc = x3d_step(c, gdt, ux, uy, uz, dux, duy, duz, pp3, px, py, pz);
+  c->timing = false;
+  if (rc != X3D_OK) return rc;
+  X3D_CHECK_HIP(hipEventSynchronize(c->pass_ev[X3D_NPASS]));
+  for (int i = 0; i < X3D_NPASS; ++i)
+    X3D_CHECK_HIP(hipEventElapsedTime(&pass_ms[i], c->pass_ev[i], c->pass_ev[i + 1]));
   return X3D_OK;
 }
 

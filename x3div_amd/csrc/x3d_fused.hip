@@ -317,8 +317,11 @@ int fused_momentum(x3d_ctx* c, double gdt, int euler, double* ux, double* uy, do
   X3D_TRY(ensure_work(c));
   double *tx = c->work[5], *ty = c->work[6], *tz = c->work[7];
   X3D_TRY(transeq_pass<0>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0));
+  X3D_TRY(mark_pass(c, 1));
   X3D_TRY(transeq_pass<1>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0));
+  X3D_TRY(mark_pass(c, 2));
   X3D_TRY(transeq_pass<2>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, euler));
+  X3D_TRY(mark_pass(c, 3));
   return X3D_OK;
 }
 
@@ -327,8 +330,11 @@ int fused_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy
   X3D_TRY(ensure_work(c));
   double *pp1 = c->work[0], *pgy1 = c->work[1], *pgz1 = c->work[2], *dd2 = c->work[3], *upi2 = c->work[4];
   X3D_TRY(div_pass<0>(c, ux, uy, uz, pp1, pgy1, pgz1));
+  X3D_TRY(mark_pass(c, 4));
   X3D_TRY(div_pass<1>(c, pp1, pgy1, pgz1, dd2, upi2, nullptr));
+  X3D_TRY(mark_pass(c, 5));
   X3D_TRY(div_pass<2>(c, dd2, upi2, nullptr, pp3, nullptr, nullptr));
+  X3D_TRY(mark_pass(c, 6));
   return X3D_OK;
 }
 
@@ -338,8 +344,11 @@ int fused_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp
   X3D_TRY(ensure_work(c));
   double *ppi3 = c->work[0], *pgz3 = c->work[1], *ppi2 = c->work[2], *pgy2 = c->work[3], *pgzi2 = c->work[4];
   X3D_TRY(grad_pass<2>(c, pp3, nullptr, nullptr, ppi3, pgz3, nullptr, nullptr, nullptr, nullptr, 0));
+  X3D_TRY(mark_pass(c, 8));
   X3D_TRY(grad_pass<1>(c, ppi3, pgz3, nullptr, ppi2, pgy2, pgzi2, nullptr, nullptr, nullptr, 0));
+  X3D_TRY(mark_pass(c, 9));
   X3D_TRY(grad_pass<0>(c, ppi2, pgy2, pgzi2, px, py, pz, ux, uy, uz, corr));
+  X3D_TRY(mark_pass(c, 10));
   return X3D_OK;
 }
 

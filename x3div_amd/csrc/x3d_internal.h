@@ -86,6 +86,9 @@ struct x3d_ctx {
   double* work[8] = {};
   size_t work_elems = 0;
   double* red = nullptr;            // reduction scratch (host-visible results)
+  // per-pass timing of x3d_step_timed: event i is recorded after pass i
+  hipEvent_t pass_ev[X3D_NPASS + 1] = {};
+  bool timing = false;
 };
 
 namespace x3d {
@@ -108,6 +111,13 @@ void poisson_release(x3d_ctx* c);
 int poisson_solve(x3d_ctx* c, double* rhs);
 
 int ensure_work(x3d_ctx* c);
+
+// Record the end of step pass i (1-based) when x3d_step_timed is running.
+inline int mark_pass(x3d_ctx* c, int i)
+{
+  if (c->timing && hipEventRecord(c->pass_ev[i], c->stream) != hipSuccess) return X3D_ERR_HIP;
+  return X3D_OK;
+}
 
 // Fused step passes (all operators must carry the library's periodic tables).
 bool fused_ok(const x3d_ctx* c);
