@@ -4,6 +4,9 @@
 // staggered routines and xthomas_0/ythomas_0/zthomas_0.
 #include <cmath>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include "x3d_line.h"
 
 namespace x3d {
@@ -45,20 +48,34 @@ Coef<double> stencil_coef(const Scheme& s, int kind)
 }
 
 template <typename T, int KIND, int M, bool CIRC, bool LF>
-__global__ __launch_bounds__(1024) void k_line(const T* in, T* out, OpDev<T> op, Coef<T> cf, Circ<T> cc,
+__global__ __launch_bounds__(LF ? 1024 : 256) void k_line(const T* in, T* out, OpDev<T> op, Coef<T> cf, Circ<T> cc,
                                                Geom geo, Tiling tl)
 {
   constexpr int HL = Halo<KIND>::L, HR = Halo<KIND>::R;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   const LineThread t = line_thread<T>(geo, tl, M);
-  T win[M + HL + HR];
-  load_window<T, M, HL, HR, LF>(in, t, geo.n, win);
   T v[M];
-  rhs_chunk<KIND, T, M, HL>(win, cf, v);
-  if constexpr (CIRC) solve_circ<T, M, LF>(v, t, cc, lds);
-  else solve_table<T, M, LF>(v, t, op, lds, geo.n, tl.fix != 0);
-  store_chunk<T, M>(out, t, v);
+  if constexpr (LF) {
+    T win[M + HL + HR];
+    load_window<T, M, HL, HR, LF>(in, t, geo.n, win);
+    rhs_chunk<KIND, T, M, HL>(win, cf, v);
+    if constexpr (CIRC) solve_circ<T, M, LF>(v, t, cc, lds);
+    else solve_table<T, M, LF>(v, t, op, lds, geo.n, tl.fix != 0);
+    store_chunk<T, M>(out, t, v);
+  } else {
+    // x-lines: the solvers use no LDS (a line is inside one wavefront), so
+    // the LDS holds the wavefronts' staging tiles
+    const XBlock<T, M> xb = x_block<T, M>(geo, t, lds);
+    {
+      T win[M + HL + HR];
+      x_load_window<T, M, HL, HR>(in, xb, t, win);
+      rhs_chunk<KIND, T, M, HL>(win, cf, v);
+    }
+    if constexpr (CIRC) solve_circ<T, M, LF>(v, t, cc, lds);
+    else solve_table<T, M, LF>(v, t, op, lds, geo.n, tl.fix != 0);
+    x_store<T, M>(out, xb, v);
+  }
 }
 
 // Points per chunk the kernels are instantiated for.
@@ -102,6 +119,20 @@ int line_plan_fused(int n, bool line_fast, LinePlan* P, int variant)
   auto okM = [](int m) { for (int mm : kMs) if (mm == m) return true; return false; };
   LinePlan p;
   p.n = n; p.line_fast = true;
+  // explicit tiling "M,G,Wl" (chunk length, chunk groups per wavefront, line
+  // groups per work-group) from X3D_FUSED_YZ, used when it fits this line
+  if (const char* e = std::getenv("X3D_FUSED_YZ")) {
+    int m = 0, g = 0, wl = 0;
+    if (std::sscanf(e, "%d,%d,%d", &m, &g, &wl) == 3 && m > 0 && okM(m) && n % m == 0 && g > 0 && g <= 64 &&
+        (64 % g) == 0 && wl > 0) {
+      const int c = n / m;
+      if (c <= 64 && c % g == 0 && (c / g) * wl <= 8) {
+        p.C = c; p.M = m; p.G = g; p.Wc = c / g; p.Wl = wl; p.LW = 64 / g;
+        *P = p;
+        return X3D_OK;
+      }
+    }
+  }
   // variant 1: 64 short chunks per line, 8 lines x 8 chunk groups per wavefront
   if (variant == 1 && n % 64 == 0 && okM(n / 64)) {
     p.C = 64; p.M = n / 64; p.G = 8; p.Wc = 8; p.Wl = 1; p.LW = 8;
@@ -135,6 +166,8 @@ void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, d
   *grid = dim3((geo->nlines + lpw - 1) / lpw);
   *block = dim3(64 * P.Wc * P.Wl);
   *smem = tbytes * (size_t)line_lds_words(P.Wc, lpw);
+  // x-lines: one staging tile of 64 rows x (M or M+1, the odd one) per wavefront
+  if (!P.line_fast) *smem = tbytes * (size_t)(P.Wc * P.Wl) * 64u * (size_t)((P.M & 1) ? P.M : P.M + 1);
 }
 
 template <typename T, int KIND, bool CIRC>

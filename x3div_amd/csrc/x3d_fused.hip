@@ -27,6 +27,8 @@ __global__ __launch_bounds__(512) void k_transeq(T* ux, T* uy, T* uz, T* dux, T*
   const LineThread t = line_thread<T, DIR>(geo, tl, M);
   const int n = geo.n;
   T* const Ud = DIR == 0 ? ux : (DIR == 1 ? uy : uz);
+  XBlock<T, M> xb;
+  if constexpr (DIR == 0) xb = x_block<T, M>(geo, t, lds);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     // components one after another: no loads hoisted across them
@@ -40,8 +42,13 @@ __global__ __launch_bounds__(512) void k_transeq(T* ux, T* uy, T* uz, T* dux, T*
     T v1[M], v2[M];
     {
       T wc[NW], wd[NW];
-      load_window<T, M, H, H, DIR != 0>(Uc, t, n, wc);
-      load_window<T, M, H, H, DIR != 0>(Ud, t, n, wd);
+      if constexpr (DIR == 0) {
+        x_load_window<T, M, H, H>(Uc, xb, t, wc);
+        x_load_window<T, M, H, H>(Ud, xb, t, wd);
+      } else {
+        load_window<T, M, H, H, DIR != 0>(Uc, t, n, wc);
+        load_window<T, M, H, H, DIR != 0>(Ud, t, n, wd);
+      }
       rhs_chunk<K_D, T, M, H>(wc, cf, v2);
 #pragma unroll
       for (int k = 0; k < M; ++k) {
@@ -57,10 +64,17 @@ __global__ __launch_bounds__(512) void k_transeq(T* ux, T* uy, T* uz, T* dux, T*
     solve_circ<T, M, DIR != 0>(v1, t, cc, lds);
     __builtin_amdgcn_sched_barrier(0);
     // u_d at the chunk points again (a cache hit) rather than kept live
-#pragma unroll
-    for (int k = 0; k < M; ++k) v1[k] = v1[k] + ld(Ud, t.at(t.j0 + k)) * v2[k];
     if constexpr (DIR == 0) {
-      store_chunk<T, M>(DUc, t, v1);
+      T ud[M];
+      x_load_chunk<T, M>(Ud, xb, ud);
+#pragma unroll
+      for (int k = 0; k < M; ++k) v1[k] = v1[k] + ud[k] * v2[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < M; ++k) v1[k] = v1[k] + ld(Ud, t.at(t.j0 + k)) * v2[k];
+    }
+    if constexpr (DIR == 0) {
+      x_store<T, M>(DUc, xb, v1);
     } else if constexpr (DIR == 1) {
       store_chunk<T, M>(TMc, t, v1);
     } else {
@@ -91,28 +105,29 @@ __global__ __launch_bounds__(512) void k_div(const T* a, const T* b, const T* c,
   const LineThread t = line_thread<T, DIR>(geo, tl, M);
   const int n = geo.n;
   if constexpr (DIR == 0) {
+    const XBlock<T, M> xb = x_block<T, M>(geo, t, lds);
     T v[M];
     {
       T w[M + 3];
-      load_window<T, M, 1, 2, DIR != 0>(a, t, n, w);
+      x_load_window<T, M, 1, 2>(a, xb, t, w);
       rhs_chunk<K_DVP, T, M, 1>(w, cfd, v);
     }
-    solve_circ<T, M, DIR != 0>(v, t, ccd, lds);
-    store_chunk<T, M>(o1, t, v);
+    solve_circ<T, M, false>(v, t, ccd, lds);
+    x_store<T, M>(o1, xb, v);
     {
       T w[M + 7];
-      load_window<T, M, 3, 4, DIR != 0>(b, t, n, w);
+      x_load_window<T, M, 3, 4>(b, xb, t, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
-    solve_circ<T, M, DIR != 0>(v, t, cci, lds);
-    store_chunk<T, M>(o2, t, v);
+    solve_circ<T, M, false>(v, t, cci, lds);
+    x_store<T, M>(o2, xb, v);
     {
       T w[M + 7];
-      load_window<T, M, 3, 4, DIR != 0>(c, t, n, w);
+      x_load_window<T, M, 3, 4>(c, xb, t, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
-    solve_circ<T, M, DIR != 0>(v, t, cci, lds);
-    store_chunk<T, M>(o3, t, v);
+    solve_circ<T, M, false>(v, t, cci, lds);
+    x_store<T, M>(o3, xb, v);
   } else if constexpr (DIR == 1) {
     T v[M], s[M];
     {
@@ -192,6 +207,7 @@ __global__ __launch_bounds__(512) void k_grad(const T* a, const T* b, const T* c
       store_chunk<T, M>(o3, t, v);
     }
   } else {
+    const XBlock<T, M> xb = x_block<T, M>(geo, t, lds);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const T* const inq = q == 0 ? a : (q == 1 ? b : c);
@@ -200,19 +216,19 @@ __global__ __launch_bounds__(512) void k_grad(const T* a, const T* b, const T* c
       T v[M];
       {
         T w[M + 7];
-        load_window<T, M, 4, 3, DIR != 0>(inq, t, n, w);
+        x_load_window<T, M, 4, 3>(inq, xb, t, w);
         if (q == 0) rhs_chunk<K_DPV, T, M, 4>(w, cfd, v);
         else rhs_chunk<K_IPV, T, M, 4>(w, cfi, v);
       }
-      if (q == 0) solve_circ<T, M, DIR != 0>(v, t, ccd, lds);
-      else solve_circ<T, M, DIR != 0>(v, t, cci, lds);
-      store_chunk<T, M>(outq, t, v);
-      if (corr && t.active) {
+      if (q == 0) solve_circ<T, M, false>(v, t, ccd, lds);
+      else solve_circ<T, M, false>(v, t, cci, lds);
+      x_store<T, M>(outq, xb, v);
+      if (corr) {
+        T u[M];
+        x_load_chunk<T, M>(Uq, xb, u);
 #pragma unroll
-        for (int k = 0; k < M; ++k) {
-          const uint32_t at = t.at(t.j0 + k);
-          st(Uq, at, ld(Uq, at) - v[k]);
-        }
+        for (int k = 0; k < M; ++k) u[k] = u[k] - v[k];
+        x_store<T, M>(Uq, xb, u);
       }
     }
   }

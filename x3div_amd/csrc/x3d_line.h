@@ -207,6 +207,129 @@ __device__ __forceinline__ void store_chunk(T* out, const LineThread& t, const T
   }
 }
 
+// ---- x-lines staged through LDS ---------------------------------------------
+// On x-lines a lane's chunk is M consecutive points, so direct chunk loads put
+// 64 lanes on addresses M*sizeof(T) apart.  Instead each wavefront moves its
+// block through LDS: its LW lines are one contiguous run of 64*M points, read
+// and written with lane-consecutive (fully coalesced) accesses and
+// redistributed to chunks in a per-wavefront LDS tile of 64 rows of P words
+// (P = M or M + 1, whichever is odd, so chunk reads do not conflict on banks).
+// Only lanes of the same wavefront touch a tile, so the hand-offs are
+// wavefront barriers, not work-group barriers.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int M>
+struct XPitch { static constexpr int P = (M & 1) ? M : M + 1; };
+
+// LDS words (of T) of the x tiles of one work-group of nwaves wavefronts.
+template <int M>
+__host__ __device__ constexpr int x_tile_words() { return 64 * XPitch<M>::P; }
+
+template <typename T, int M>
+struct XBlock {
+  T* tile;        // this wavefront's LDS tile [64][P]
+  uint32_t base;  // byte offset of the block's first point
+  uint32_t lim;   // byte offset of the field's last point (clamp for partial blocks)
+  int valid;      // points of the block that belong to existing lines
+  int lane;
+};
+
+// tiles: the work-group's x tiles, one per wavefront, in wavefront order.
+template <typename T, int M>
+__device__ __forceinline__ XBlock<T, M> x_block(const Geom& geo, const LineThread& t, T* tiles)
+{
+  XBlock<T, M> b;
+  const int wave = threadIdx.x >> 6;
+  b.lane = threadIdx.x & 63;
+  b.tile = tiles + wave * x_tile_words<M>();
+  const int first = blockIdx.x * t.LPW + (t.ls - t.lineLocal);  // first line of this wavefront
+  const long long left = (long long)(geo.nlines - first) * geo.n;
+  b.valid = left <= 0 ? 0 : (left >= 64 * M ? 64 * M : (int)left);
+  b.base = (uint32_t)first * (uint32_t)geo.n * (uint32_t)sizeof(T);
+  b.lim = ((uint32_t)geo.nlines * (uint32_t)geo.n - 1u) * (uint32_t)sizeof(T);
+  return b;
+}
+
+// Coalesced load of the block into the tile (point e of the block goes to
+// row e / M, column e % M).
+template <typename T, int M>
+__device__ __forceinline__ void x_stage_in(const T* __restrict__ in, const XBlock<T, M>& b)
+{
+  constexpr int P = XPitch<M>::P;
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int e = b.lane + 64 * k;
+    uint32_t off = b.base + (uint32_t)e * (uint32_t)sizeof(T);
+    off = off < b.lim ? off : b.lim;
+    b.tile[(e / M) * P + e % M] = ld(in, off);
+  }
+  wave_lds_sync();
+}
+
+// Window j0-HL .. j0+M+HR-1 of this lane's chunk from the staged tile; the
+// halo is in the neighbouring chunks of the same line (wrapped mod n).
+template <typename T, int M, int HL, int HR>
+__device__ __forceinline__ void x_window(const XBlock<T, M>& b, const LineThread& t, T (&win)[M + HL + HR])
+{
+  constexpr int P = XPitch<M>::P;
+  const T* own = b.tile + b.lane * P;
+#pragma unroll
+  for (int k = 0; k < M; ++k) win[HL + k] = own[k];
+#pragma unroll
+  for (int h = 1; h <= HL; ++h) {
+    const int d = (h + M - 1) / M;
+    const int src = t.lane_of((t.g - d % t.G + t.G) % t.G);
+    win[HL - h] = b.tile[src * P + (M * d - h)];
+  }
+#pragma unroll
+  for (int h = 1; h <= HR; ++h) {
+    const int d = (h + M - 1) / M;
+    const int src = t.lane_of((t.g + d) % t.G);
+    win[HL + M - 1 + h] = b.tile[src * P + (h - 1 - M * (d - 1))];
+  }
+}
+
+template <typename T, int M, int HL, int HR>
+__device__ __forceinline__ void x_load_window(const T* __restrict__ in, const XBlock<T, M>& b,
+                                              const LineThread& t, T (&win)[M + HL + HR])
+{
+  x_stage_in<T, M>(in, b);
+  x_window<T, M, HL, HR>(b, t, win);
+}
+
+// This lane's chunk of a field, through the tile.
+template <typename T, int M>
+__device__ __forceinline__ void x_load_chunk(const T* __restrict__ in, const XBlock<T, M>& b, T (&v)[M])
+{
+  x_stage_in<T, M>(in, b);
+  const T* own = b.tile + b.lane * XPitch<M>::P;
+#pragma unroll
+  for (int k = 0; k < M; ++k) v[k] = own[k];
+}
+
+// Chunk values to the tile, then coalesced stores of the block's valid points.
+template <typename T, int M>
+__device__ __forceinline__ void x_store(T* out, const XBlock<T, M>& b, const T (&v)[M])
+{
+  constexpr int P = XPitch<M>::P;
+  wave_lds_sync();
+  T* own = b.tile + b.lane * P;
+#pragma unroll
+  for (int k = 0; k < M; ++k) own[k] = v[k];
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int e = b.lane + 64 * k;
+    if (e < b.valid) st(out, b.base + (uint32_t)e * (uint32_t)sizeof(T), b.tile[(e / M) * P + e % M]);
+  }
+}
+
 // Right-hand side of KIND for the M chunk points from a window whose halo is
 // HLW points on the left (HLW >= Halo<KIND>::L).
 template <int KIND, typename T, int M, int HLW, int NW>
