@@ -103,6 +103,69 @@ def cpu_baseline(n: int):
     }
 
 
+def rel_errs(a, b):
+    """max and L2 norms of a - b relative to b, in FP64."""
+    a = a.double()
+    d = (a - b).abs()
+    return {"max": float(d.max() / b.abs().max()), "l2": float((a - b).norm() / b.norm())}
+
+
+def time_form(ctx, f, step_fn, timed_fn, args):
+    """W untimed steps, K steps timed between device synchronisations (wall
+    clock and HIP events on the library's stream), then per-pass times of
+    kernel_reps more steps from events recorded between the passes."""
+    import torch
+    for _ in range(args.warmup):
+        step_fn(*f, gdt=0.0)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(ctx.stream)
+    for _ in range(args.steps):
+        step_fn(*f, gdt=0.0)
+    ev1.record(ctx.stream)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    gpu_ms_step = ev0.elapsed_time(ev1) / args.steps
+    acc = {}
+    for _ in range(args.kernel_reps):
+        for name, (ms, bpp) in timed_fn(*f, gdt=0.0).items():
+            acc.setdefault(name, [0.0, bpp])[0] += ms
+    passes = {}
+    n3 = ctx.nx * ctx.ny * ctx.nz
+    for name, (ms, bpp) in acc.items():
+        ms /= args.kernel_reps
+        passes[name] = {"ms": round(ms, 4), "bytes_per_point": bpp,
+                        "GBps": round(bpp * n3 / (ms / 1e3) / 1e9, 1)}
+    return ms_step, gpu_ms_step, passes
+
+
+def roofline_of(passes, n, gpu_ms_step, suffix):
+    """The slowest line pass (the Poisson pass is several hipFFT kernels and is
+    reported beside it) against the HBM peak, plus the whole step's fraction."""
+    dom = max((k for k in passes if k != "poisson"), key=lambda k: passes[k]["ms"])
+    kms = passes[dom]["ms"]
+    alg_bytes = int(passes[dom]["bytes_per_point"] * n ** 3)
+    achieved = alg_bytes / (kms / 1e3) / 1e9
+    step_bytes = sum(p["bytes_per_point"] for p in passes.values()) * n ** 3
+    kern = KERNEL_OF_PASS[dom].replace("double", "float") if suffix else KERNEL_OF_PASS[dom]
+    return {
+        "bound": "hbm",
+        "kernel": f"{dom} ({kern})",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profile(dom + suffix, n),
+        "avg_launch_ms": round(kms, 4),
+        "algorithmic_bytes_per_launch": alg_bytes,
+        "step_algorithmic_bytes": int(step_bytes),
+        "step_frac": round(step_bytes / (gpu_ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "passes": passes,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -122,62 +185,48 @@ def main():
     L = 2 * np.pi
     ctx = X3D(n, n, n, L, L, L)
     f = [ctx.field() for _ in range(10)]
-    ux, uy, uz, dux, duy, duz, pp3, px, py, pz = f
-    ctx.init_fields(ux, uy, uz, kind=1, seed=1234, amp=1e-3)
-    stream = ctx.stream
+    ctx.init_fields(f[0], f[1], f[2], kind=1, seed=1234, amp=1e-3)
+    u0 = [t.clone() for t in f[:3]]
 
-    def step():
-        ctx.step(ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt=0.0)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = wall * 1e3 / args.steps
-    gpu_ms_step = ev0.elapsed_time(ev1) / args.steps
+    # plain FP64 form: the drop-in, and the headline value
+    ms_step, gpu_ms_step, passes = time_form(ctx, f, ctx.step, ctx.step_timed, args)
     value = n ** 3 / (ms_step / 1e3) / 1e6
+    roofline = roofline_of(passes, n, gpu_ms_step, "")
 
-    # per-pass times of the same step, HIP events on the library's stream; the
-    # dominant kernel is the slowest line pass (the Poisson pass is several
-    # hipFFT kernels and is reported beside it)
-    acc = {}
-    for _ in range(args.kernel_reps):
-        for name, (ms, bpp) in ctx.step_timed(ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt=0.0).items():
-            acc.setdefault(name, [0.0, bpp])[0] += ms
-    passes = {}
-    for name, (ms, bpp) in acc.items():
-        ms /= args.kernel_reps
-        passes[name] = {"ms": round(ms, 4), "bytes_per_point": bpp,
-                        "GBps": round(bpp * n ** 3 / (ms / 1e3) / 1e9, 1)}
-    dom = max((k for k in passes if k != "poisson"), key=lambda k: passes[k]["ms"])
-    kms = passes[dom]["ms"]
-    alg_bytes = int(passes[dom]["bytes_per_point"] * n ** 3)
-    achieved = alg_bytes / (kms / 1e3) / 1e9
-    step_bytes = sum(p["bytes_per_point"] for p in passes.values()) * n ** 3
-    roofline = {
-        "bound": "hbm",
-        "kernel": f"{dom} ({KERNEL_OF_PASS[dom]})",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profile(dom, n),
-        "avg_launch_ms": round(kms, 4),
-        "algorithmic_bytes_per_launch": alg_bytes,
-        "step_algorithmic_bytes": int(step_bytes),
-        "step_frac": round(step_bytes / (gpu_ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-        "passes": passes,
+    # precision form: FP32 fields and line solves, FP64 spectral division,
+    # started from the same fields rounded to FP32
+    f32 = [t.float() for t in u0] + [ctx.field32() for _ in range(7)]
+    ms32, gpu_ms32, passes32 = time_form(ctx, f32, ctx.step_f32, ctx.step_timed_f32, args)
+    # accuracy of one step and of the 18 operators against the plain form
+    # (itself within 1e-12 of the CPU oracle)
+    g64 = [t.clone() for t in u0] + [ctx.field() for _ in range(7)]
+    g32 = [t.float() for t in u0] + [ctx.field32() for _ in range(7)]
+    ctx.step(*g64, gdt=0.0)
+    ctx.step_f32(*g32, gdt=0.0)
+    torch.cuda.synchronize()
+    step_err = {name: rel_errs(g32[i], g64[i]) for i, name in ((0, "ux"), (3, "dux"), (6, "pp3"))}
+    del g64, g32
+    op_err = {}
+    o64, o32, i32 = ctx.field(), ctx.field32(), u0[0].float()
+    from x3div_amd import OPS
+    for name in OPS:
+        ctx.apply(name, o64, u0[0])
+        ctx.apply_f32(name, o32, i32)
+        torch.cuda.synchronize()
+        op_err[name] = rel_errs(o32, o64)["max"]
+    del o64, o32, i32
+    forms = {
+        "fp64": {"value": round(value, 2), "ms_per_step": round(ms_step, 4), "dtype": "f64",
+                 "step_frac": roofline["step_frac"]},
+        "fp32": {"value": round(n ** 3 / (ms32 / 1e3) / 1e6, 2), "ms_per_step": round(ms32, 4),
+                 "dtype": "f32 line solves and FFTs, f64 spectral division",
+                 "speedup_vs_fp64": round(ms_step / ms32, 3),
+                 "roofline": roofline_of(passes32, n, gpu_ms32, "_f32"),
+                 "step_rel_err_vs_fp64": step_err,
+                 "operator_max_rel_err_vs_fp64": op_err},
     }
     ctx.close()
-    del f
+    del f, f32, u0
     torch.cuda.empty_cache()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_n)
@@ -203,6 +252,7 @@ def main():
             "parallelism": "1 GPU (prow=pcol=1)",
         },
         "roofline": roofline,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

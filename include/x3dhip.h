@@ -184,6 +184,68 @@ int x3d_step_timed(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
 const char *x3d_pass_name(int pass);
 double x3d_pass_bytes_per_point(int pass);
 
+/* ---- precision form: FP32 fields, FP32 line solves ------------------------
+ * The same operators and step as above on FP32 arrays.  The line solves run
+ * in FP32 (circulant solver on the library's periodic tables; an operator
+ * replaced with x3d_operator_set gives X3D_ERR_UNSUPPORTED).  Poisson runs
+ * FP32 FFTs around an FP64 modified-wavenumber division.  Results differ from
+ * the FP64 entry points by FP32 rounding (about 1e-7 relative for one
+ * derivative). */
+int x3d_derx_f32(x3d_ctx *ctx, float *tx, const float *ux, int op, int nx, int ny, int nz);
+int x3d_dery_f32(x3d_ctx *ctx, float *ty, const float *uy, int op, const float *ppy, int nx, int ny, int nz);
+int x3d_derz_f32(x3d_ctx *ctx, float *tz, const float *uz, int op, int nx, int ny, int nz);
+/* Any of the 18 operators along its own direction: derivatives (op 0-5), the
+ * staggered derivatives and interpolations (op 6-17), as the FP64 routines
+ * of src/x3d_derive.f90 and src/x3d_staggered.f90 with periodic BCs. */
+int x3d_apply_f32(x3d_ctx *ctx, int op, float *out, const float *in);
+int x3d_poisson_f32(x3d_ctx *ctx, float *rhs);
+int x3d_step_f32(x3d_ctx *ctx, double gdt, float *ux, float *uy, float *uz,
+                 float *dux, float *duy, float *duz, float *pp3,
+                 float *px, float *py, float *pz);
+int x3d_step_timed_f32(x3d_ctx *ctx, double gdt, float *ux, float *uy, float *uz,
+                       float *dux, float *duy, float *duz, float *pp3,
+                       float *px, float *py, float *pz, float *pass_ms);
+
+/* ---- captured step --------------------------------------------------------
+ * Record one x3d_step (or x3d_step_f32) with these arrays into a HIP graph
+ * held by the context (replacing any earlier one); x3d_graph_launch replays
+ * it on the context's stream.  Scratch memory and FFT plans are allocated
+ * before the capture, so replays allocate nothing.  The reference drives the
+ * step loop from the host, one call at a time (src/xcompact3d.f90:74-104). */
+int x3d_graph_capture_step(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
+                           double *dux, double *duy, double *duz, double *pp3,
+                           double *px, double *py, double *pz);
+int x3d_graph_capture_step_f32(x3d_ctx *ctx, double gdt, float *ux, float *uy, float *uz,
+                               float *dux, float *duy, float *duz, float *pp3,
+                               float *px, float *py, float *pz);
+int x3d_graph_launch(x3d_ctx *ctx);
+
+/* ---- batching form ---------------------------------------------------------
+ * Many independent triply periodic grids of uneven sizes, each the
+ * reference's problem on its own (nx, ny, nz) (src/xcompact3d.f90:80-95),
+ * stepped together.  The batch owns the fields: per grid the 10 arrays
+ * ux uy uz dux duy duz pp3 px py pz (field index 0-9), of FP64 (precision
+ * 8) or FP32 (precision 4) values.  Each step pass is one persistent launch
+ * per group of grids sharing a line tiling; Poisson runs per grid.  A grid's
+ * own context (x3d_batch_context) serves the single-grid entry points.
+ * dims: nx, ny, nz per grid; lengths: xlx, yly, zlz per grid, or NULL for
+ * 2*pi.  Seeds of x3d_batch_init_fields are seed + grid index. */
+typedef struct x3d_batch x3d_batch;
+int x3d_batch_create(int ngrids, const int *dims, const double *lengths, int precision, x3d_batch **batch);
+int x3d_batch_destroy(x3d_batch *batch);
+int x3d_batch_set_stream(x3d_batch *batch, void *hip_stream);
+int x3d_batch_sizes(const x3d_batch *batch, int *ngrids, long long *total_points);
+int x3d_batch_field(const x3d_batch *batch, int grid, int field, void **dptr);
+int x3d_batch_context(const x3d_batch *batch, int grid, x3d_ctx **ctx);
+int x3d_batch_init_fields(x3d_batch *batch, int kind, uint64_t seed, double amp);
+int x3d_batch_step(x3d_batch *batch, double gdt);
+/* Record one batched step into a HIP graph (gdt fixed at capture); replay it. */
+int x3d_batch_graph_capture(x3d_batch *batch, double gdt);
+int x3d_batch_graph_launch(x3d_batch *batch);
+/* Time the batched step (reps steps per trial) for every y/z tiling that fits
+ * each line length, keep the fastest, and write a text report. */
+int x3d_batch_autotune(x3d_batch *batch, int reps, char *report, size_t report_len);
+
 /* ---- test mode and fields --------------------------------------------- */
 /* test_du/test_dv/test_dw RMS error vs the analytic derivative of vel()
  * (src/mom.f90:59-243); which = 0,1,2.  Synchronises. */

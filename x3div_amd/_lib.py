@@ -69,6 +69,27 @@ SIGNATURES = {
     "x3d_solve_poisson": (_i, [_vp] + [_dp] * 7),
     "x3d_step": (_i, [_vp, _d] + [_dp] * 10),
     "x3d_step_timed": (_i, [_vp, _d] + [_dp] * 10 + [C.POINTER(C.c_float)]),
+    "x3d_derx_f32": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
+    "x3d_dery_f32": (_i, [_vp, _dp, _dp, _i, _dp, _i, _i, _i]),
+    "x3d_derz_f32": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
+    "x3d_apply_f32": (_i, [_vp, _i, _dp, _dp]),
+    "x3d_poisson_f32": (_i, [_vp, _dp]),
+    "x3d_step_f32": (_i, [_vp, _d] + [_dp] * 10),
+    "x3d_step_timed_f32": (_i, [_vp, _d] + [_dp] * 10 + [C.POINTER(C.c_float)]),
+    "x3d_graph_capture_step": (_i, [_vp, _d] + [_dp] * 10),
+    "x3d_graph_capture_step_f32": (_i, [_vp, _d] + [_dp] * 10),
+    "x3d_graph_launch": (_i, [_vp]),
+    "x3d_batch_create": (_i, [_i, C.POINTER(C.c_int), C.POINTER(C.c_double), _i, C.POINTER(_vp)]),
+    "x3d_batch_destroy": (_i, [_vp]),
+    "x3d_batch_set_stream": (_i, [_vp, _vp]),
+    "x3d_batch_sizes": (_i, [_vp, _ip, C.POINTER(C.c_longlong)]),
+    "x3d_batch_field": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
+    "x3d_batch_context": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "x3d_batch_init_fields": (_i, [_vp, _i, C.c_uint64, _d]),
+    "x3d_batch_step": (_i, [_vp, _d]),
+    "x3d_batch_graph_capture": (_i, [_vp, _d]),
+    "x3d_batch_graph_launch": (_i, [_vp]),
+    "x3d_batch_autotune": (_i, [_vp, _i, C.c_char_p, _sz]),
     "x3d_pass_name": (C.c_char_p, [_i]),
     "x3d_pass_bytes_per_point": (_d, [_i]),
     "x3d_test_err": (_i, [_vp, _dp, _i, _hp]),

@@ -32,14 +32,15 @@ def _op_id(op) -> int:
     return int(op)
 
 
-def _ptr(t, shape=None) -> int:
+def _ptr(t, shape=None, dtype=None) -> int:
     import torch
+    dtype = dtype or torch.float64
     if not isinstance(t, torch.Tensor):
         raise TypeError("expected a torch tensor on the GPU")
     if not t.is_cuda:
         raise ValueError("x3div_amd operates on device tensors only (no CPU fallback)")
-    if t.dtype != torch.float64:
-        raise TypeError(f"expected float64, got {t.dtype}")
+    if t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
     if shape is not None and tuple(t.shape) != tuple(shape):
@@ -95,6 +96,18 @@ class X3D:
 
     def _f(self, t):
         return _ptr(t, self.shape)
+
+    def field32(self, fill: float | None = 0.0):
+        """An FP32 field for the precision form."""
+        import torch
+        t = torch.empty(self.shape, dtype=torch.float32, device="cuda")
+        if fill is not None:
+            t.fill_(fill)
+        return t
+
+    def _f32(self, t):
+        import torch
+        return _ptr(t, self.shape, torch.float32)
 
     # ---- operator tables ----------------------------------------------------
     def operator(self, op):
@@ -215,6 +228,54 @@ class X3D:
     def poisson(self, rhs):
         check("x3d_poisson", self.lib.x3d_poisson(self._h, self._f(rhs)))
 
+    # ---- captured step ------------------------------------------------------------
+    def graph_capture_step(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """Record one step on these fields into the context's HIP graph."""
+        check("x3d_graph_capture_step", self.lib.x3d_graph_capture_step(
+            self._h, float(gdt), *(self._f(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
+
+    def graph_capture_step_f32(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        check("x3d_graph_capture_step_f32", self.lib.x3d_graph_capture_step_f32(
+            self._h, float(gdt), *(self._f32(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
+
+    def graph_launch(self):
+        """Replay the captured step on the context's stream."""
+        check("x3d_graph_launch", self.lib.x3d_graph_launch(self._h))
+
+    # ---- precision form (FP32 fields) ------------------------------------------------
+    def derx_f32(self, tx, ux, op="derx"):
+        check("x3d_derx_f32", self.lib.x3d_derx_f32(self._h, self._f32(tx), self._f32(ux), _op_id(op),
+                                                    self.nx, self.ny, self.nz))
+
+    def dery_f32(self, ty, uy, op="dery"):
+        check("x3d_dery_f32", self.lib.x3d_dery_f32(self._h, self._f32(ty), self._f32(uy), _op_id(op), None,
+                                                    self.nx, self.ny, self.nz))
+
+    def derz_f32(self, tz, uz, op="derz"):
+        check("x3d_derz_f32", self.lib.x3d_derz_f32(self._h, self._f32(tz), self._f32(uz), _op_id(op),
+                                                    self.nx, self.ny, self.nz))
+
+    def apply_f32(self, name: str, out, inp):
+        """Any of the 18 operators, by instance name, along its own direction, in FP32."""
+        check("x3d_apply_f32", self.lib.x3d_apply_f32(self._h, _op_id(name), self._f32(out), self._f32(inp)))
+
+    def poisson_f32(self, rhs):
+        check("x3d_poisson_f32", self.lib.x3d_poisson_f32(self._h, self._f32(rhs)))
+
+    def step_f32(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """The step of the precision form on FP32 fields."""
+        check("x3d_step_f32", self.lib.x3d_step_f32(
+            self._h, float(gdt), *(self._f32(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
+
+    def step_timed_f32(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """step_timed for the precision form; bytes per point are half the FP64 figures."""
+        from ._lib import X3D_NPASS
+        ms = (C.c_float * X3D_NPASS)()
+        check("x3d_step_timed_f32", self.lib.x3d_step_timed_f32(
+            self._h, float(gdt), *(self._f32(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz)), ms))
+        return {self.lib.x3d_pass_name(i).decode(): (ms[i], self.lib.x3d_pass_bytes_per_point(i) / 2)
+                for i in range(X3D_NPASS)}
+
     def poisson_tables(self):
         hp = C.POINTER(C.c_double)
         a = [np.zeros(n) for n in (self.nx, self.nx, self.ny, self.ny, self.nz, self.nz)]
@@ -292,3 +353,91 @@ class X3D:
 
 def _axis(name: str) -> int:
     return "xyz".index(name[3])
+
+
+class X3DBatch:
+    """Many independent periodic grids stepped together (the batching form).
+
+    dims: list of (nx, ny, nz); lengths: list of (xlx, yly, zlz) or None for
+    2*pi; precision: 8 (FP64) or 4 (FP32).  The library owns the fields; they
+    are read and written through host copies (get / put)."""
+
+    FIELDS = ["ux", "uy", "uz", "dux", "duy", "duz", "pp3", "px", "py", "pz"]
+
+    def __init__(self, dims, lengths=None, precision: int = 8, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("x3div_amd needs a HIP device (no CPU fallback)")
+        self.lib = _lib.load()
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        self.precision = precision
+        flat = (C.c_int * (3 * len(self.dims)))(*[v for d in self.dims for v in d])
+        lp = None
+        if lengths is not None:
+            lp = (C.c_double * (3 * len(self.dims)))(*[float(v) for d in lengths for v in d])
+        h = C.c_void_p()
+        check("x3d_batch_create", self.lib.x3d_batch_create(len(self.dims), flat, lp, precision, C.byref(h)))
+        self._h = h
+        st = stream if stream is not None else torch.cuda.current_stream()
+        check("x3d_batch_set_stream", self.lib.x3d_batch_set_stream(self._h, C.c_void_p(st.cuda_stream)))
+        self.stream = st
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.x3d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def total_points(self) -> int:
+        t = C.c_longlong()
+        check("x3d_batch_sizes", self.lib.x3d_batch_sizes(self._h, None, C.byref(t)))
+        return t.value
+
+    def _ptr(self, grid: int, name: str):
+        p = C.c_void_p()
+        check("x3d_batch_field", self.lib.x3d_batch_field(self._h, grid, self.FIELDS.index(name), C.byref(p)))
+        return p
+
+    def _ctx(self, grid: int):
+        c = C.c_void_p()
+        check("x3d_batch_context", self.lib.x3d_batch_context(self._h, grid, C.byref(c)))
+        return c
+
+    def _dtype(self):
+        return np.float64 if self.precision == 8 else np.float32
+
+    def get(self, grid: int, name: str) -> np.ndarray:
+        nx, ny, nz = self.dims[grid]
+        out = np.empty((nz, ny, nx), dtype=self._dtype())
+        check("x3d_memcpy_d2h", self.lib.x3d_memcpy_d2h(self._ctx(grid), out.ctypes.data_as(C.c_void_p),
+                                                         self._ptr(grid, name), out.nbytes))
+        return out
+
+    def put(self, grid: int, name: str, a) -> None:
+        nx, ny, nz = self.dims[grid]
+        a = np.ascontiguousarray(a, dtype=self._dtype()).reshape(nz, ny, nx)
+        check("x3d_memcpy_h2d", self.lib.x3d_memcpy_h2d(self._ctx(grid), self._ptr(grid, name),
+                                                         a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def init_fields(self, kind: int = 2, seed: int = 1234, amp: float = 0.0):
+        check("x3d_batch_init_fields", self.lib.x3d_batch_init_fields(self._h, kind, seed, amp))
+
+    def step(self, gdt: float = 0.0):
+        check("x3d_batch_step", self.lib.x3d_batch_step(self._h, float(gdt)))
+
+    def graph_capture(self, gdt: float = 0.0):
+        check("x3d_batch_graph_capture", self.lib.x3d_batch_graph_capture(self._h, float(gdt)))
+
+    def graph_launch(self):
+        check("x3d_batch_graph_launch", self.lib.x3d_batch_graph_launch(self._h))
+
+    def autotune(self, reps: int = 3) -> str:
+        buf = C.create_string_buffer(8192)
+        check("x3d_batch_autotune", self.lib.x3d_batch_autotune(self._h, reps, buf, len(buf)))
+        return buf.value.decode()

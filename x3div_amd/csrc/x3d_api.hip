@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 
 #include "x3d_internal.h"
 
@@ -382,6 +383,11 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   // tiling variant of the fused y/z passes (X3D_FUSED_TILING=0|1, default 0)
   const char* tv = std::getenv("X3D_FUSED_TILING");
   const int variant = tv ? std::atoi(tv) : 0;
+  // FP32 precision-form plans; a line they cannot tile leaves M = 0 and the
+  // FP32 step reports X3D_ERR_UNSUPPORTED
+  if (line_plan_fused32(nx, false, &c->fplan32[0]) || line_plan_fused32(ny, true, &c->fplan32[1]) ||
+      line_plan_fused32(nz, true, &c->fplan32[2]))
+    c->fplan32[0].M = c->fplan32[1].M = c->fplan32[2].M = 0;
   if ((rc = line_plan(nx, false, &c->plan[0])) || (rc = line_plan(ny, true, &c->plan[1])) ||
       (rc = line_plan(nz, true, &c->plan[2])) || (rc = line_plan_fused(nx, false, &c->fplan[0], variant)) ||
       (rc = line_plan_fused(ny, true, &c->fplan[1], variant)) ||
@@ -394,6 +400,8 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   return X3D_OK;
 }
 
+static void graph_release(x3d_ctx* c);
+
 int x3d_destroy(x3d_ctx* c)
 {
   if (!c) return X3D_ERR_ARG;
@@ -405,6 +413,7 @@ int x3d_destroy(x3d_ctx* c)
   if (c->red) (void)hipFree(c->red);
   for (hipEvent_t e : c->pass_ev)
     if (e) (void)hipEventDestroy(e);
+  graph_release(c);
   delete c;
   return X3D_OK;
 }
@@ -696,7 +705,7 @@ int x3d_momentum_rhs(x3d_ctx* c, double* dux, double* duy, double* duz, const do
   if (!c || !dux || !duy || !duz || !ux || !uy || !uz) return X3D_ERR_ARG;
   if (!fused_ok(c)) return ref_momentum_rhs(c, dux, duy, duz, ux, uy, uz);
   // the fused passes do not write u when euler = 0
-  return fused_momentum(c, 0.0, 0, const_cast<double*>(ux), const_cast<double*>(uy),
+  return fused_momentum<double>(c, 0.0, 0, const_cast<double*>(ux), const_cast<double*>(uy),
                         const_cast<double*>(uz), dux, duy, duz);
 }
 
@@ -714,14 +723,14 @@ int x3d_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, 
 {
   if (!c || !pp3 || !ux || !uy || !uz) return X3D_ERR_ARG;
   if (!fused_ok(c)) return ref_divergence(c, pp3, ux, uy, uz);
-  return fused_divergence(c, pp3, ux, uy, uz);
+  return fused_divergence<double>(c, pp3, ux, uy, uz);
 }
 
 int x3d_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3)
 {
   if (!c || !px || !py || !pz || !pp3) return X3D_ERR_ARG;
   if (!fused_ok(c)) return ref_gradp(c, px, py, pz, pp3);
-  return fused_gradp(c, px, py, pz, pp3, nullptr, nullptr, nullptr, 0);
+  return fused_gradp<double>(c, px, py, pz, pp3, nullptr, nullptr, nullptr, 0);
 }
 
 int x3d_cor_vel(x3d_ctx* c, double* ux, double* uy, double* uz, const double* px, const double* py,
@@ -756,11 +765,11 @@ int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double*
   }
   // transeq with the Euler update folded into its z pass, divergence, Poisson,
   // gradp with cor_vel folded into its x pass
-  X3D_TRY(fused_momentum(c, gdt, 1, ux, uy, uz, dux, duy, duz));
-  X3D_TRY(fused_divergence(c, pp3, ux, uy, uz));
+  X3D_TRY(fused_momentum<double>(c, gdt, 1, ux, uy, uz, dux, duy, duz));
+  X3D_TRY(fused_divergence<double>(c, pp3, ux, uy, uz));
   X3D_TRY(poisson_solve(c, pp3));
   X3D_TRY(mark_pass(c, 7));
-  X3D_TRY(fused_gradp(c, px, py, pz, pp3, ux, uy, uz, 1));
+  X3D_TRY(fused_gradp<double>(c, px, py, pz, pp3, ux, uy, uz, 1));
   return X3D_OK;
 }
 
@@ -777,6 +786,161 @@ constexpr PassInfo kPass[X3D_NPASS] = {
     {"transeq_x", 6}, {"transeq_y", 6}, {"transeq_z", 15}, {"divergence_x", 6}, {"divergence_y", 5},
     {"divergence_z", 3}, {"poisson", 6}, {"gradp_z", 3}, {"gradp_y", 5}, {"gradp_x", 12}};
 }  // namespace
+
+// ---- precision form (FP32 fields, FP32 line solves) -------------------------
+int x3d_derx_f32(x3d_ctx* c, float* t, const float* u, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line_f32(c, 0, K_D, op, u, t);
+}
+int x3d_dery_f32(x3d_ctx* c, float* t, const float* u, int op, const float* ppy, int nx, int ny, int nz)
+{
+  (void)ppy;
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line_f32(c, 1, K_D, op, u, t);
+}
+int x3d_derz_f32(x3d_ctx* c, float* t, const float* u, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line_f32(c, 2, K_D, op, u, t);
+}
+
+int x3d_apply_f32(x3d_ctx* c, int op, float* out, const float* in)
+{
+  if (!c || op < 0 || op >= X3D_NOPS) return X3D_ERR_ARG;
+  // operator ids: 0-5 D (x, x, y, y, z, z); then Dvp, Dpv, Ivp, Ipv, each x, y, z
+  const int dir = op < 6 ? op / 2 : (op - 6) % 3;
+  static const int kinds[4] = {K_DVP, K_DPV, K_IVP, K_IPV};
+  const int kind = op < 6 ? K_D : kinds[(op - 6) / 3];
+  return launch_line_f32(c, dir, kind, op, in, out);
+}
+
+int x3d_poisson_f32(x3d_ctx* c, float* rhs)
+{
+  if (!c || !rhs) return X3D_ERR_ARG;
+  return poisson_solve_f32(c, rhs);
+}
+
+static bool fp32_ok(const x3d_ctx* c)
+{
+  return fused_ok(c) && c->fplan32[0].M && c->fplan32[1].M && c->fplan32[2].M;
+}
+
+int x3d_step_f32(x3d_ctx* c, double gdt, float* ux, float* uy, float* uz, float* dux, float* duy, float* duz,
+                 float* pp3, float* px, float* py, float* pz)
+{
+  if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  if (!fp32_ok(c)) return X3D_ERR_UNSUPPORTED;
+  X3D_TRY(fused_momentum<float>(c, (float)gdt, 1, ux, uy, uz, dux, duy, duz));
+  X3D_TRY(fused_divergence<float>(c, pp3, ux, uy, uz));
+  X3D_TRY(poisson_solve_f32(c, pp3));
+  X3D_TRY(mark_pass(c, 7));
+  X3D_TRY(fused_gradp<float>(c, px, py, pz, pp3, ux, uy, uz, 1));
+  return X3D_OK;
+}
+
+int x3d_step_timed_f32(x3d_ctx* c, double gdt, float* ux, float* uy, float* uz, float* dux, float* duy,
+                       float* duz, float* pp3, float* px, float* py, float* pz, float* pass_ms)
+{
+  if (!c || !pass_ms) return X3D_ERR_ARG;
+  if (!fp32_ok(c)) return X3D_ERR_UNSUPPORTED;
+  for (int i = 0; i <= X3D_NPASS; ++i)
+    if (!c->pass_ev[i]) X3D_CHECK_HIP(hipEventCreate(&c->pass_ev[i]));
+  X3D_CHECK_HIP(hipEventRecord(c->pass_ev[0], c->stream));
+  c->timing = true;
+  const int rc = x3d_step_f32(c, gdt, ux, uy, uz, dux, duy, duz, pp3, px, py, pz);
+  c->timing = false;
+  if (rc != X3D_OK) return rc;
+  X3D_CHECK_HIP(hipEventSynchronize(c->pass_ev[X3D_NPASS]));
+  for (int i = 0; i < X3D_NPASS; ++i)
+    X3D_CHECK_HIP(hipEventElapsedTime(&pass_ms[i], c->pass_ev[i], c->pass_ev[i + 1]));
+  return X3D_OK;
+}
+
+// ---- captured step ----------------------------------------------------------
+static void graph_release(x3d_ctx* c)
+{
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
+static int fft_streams(x3d_ctx* c, hipStream_t s)
+{
+  if (c->fft_ready && (hipfftSetStream(c->fft_r2c, s) != HIPFFT_SUCCESS ||
+                       hipfftSetStream(c->fft_c2r, s) != HIPFFT_SUCCESS))
+    return X3D_ERR_FFT;
+  if (c->fft32_ready && (hipfftSetStream(c->fft_r2c32, s) != HIPFFT_SUCCESS ||
+                         hipfftSetStream(c->fft_c2r32, s) != HIPFFT_SUCCESS))
+    return X3D_ERR_FFT;
+  return X3D_OK;
+}
+
+// Record body's launches on a private stream into a graph; every allocation
+// (scratch fields, FFT plans) is made before the capture starts.
+static int capture_graph(x3d_ctx* c, const std::function<int()>& body)
+{
+  graph_release(c);
+  hipStream_t cs = nullptr;
+  X3D_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipStream_t user = c->stream;
+  c->stream = cs;
+  int rc = fft_streams(c, cs);
+  hipGraph_t g = nullptr;
+  if (!rc) {
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = X3D_ERR_HIP;
+    } else {
+      rc = body();
+      const hipError_t e = hipStreamEndCapture(cs, &g);
+      if (!rc && e != hipSuccess) rc = hip_status(e);
+    }
+  }
+  c->stream = user;
+  const int rs = fft_streams(c, user);
+  (void)hipStreamDestroy(cs);
+  if (!rc) rc = rs;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  const hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    c->graph_exec = nullptr;
+    return hip_status(e);
+  }
+  c->graph = g;
+  return X3D_OK;
+}
+
+int x3d_graph_capture_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double* dux,
+                           double* duy, double* duz, double* pp3, double* px, double* py, double* pz)
+{
+  if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  if (!fused_ok(c)) return X3D_ERR_UNSUPPORTED;
+  X3D_TRY(ensure_work(c));
+  return capture_graph(c, [&] { return x3d_step(c, gdt, ux, uy, uz, dux, duy, duz, pp3, px, py, pz); });
+}
+
+int x3d_graph_capture_step_f32(x3d_ctx* c, double gdt, float* ux, float* uy, float* uz, float* dux,
+                               float* duy, float* duz, float* pp3, float* px, float* py, float* pz)
+{
+  if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  if (!fp32_ok(c)) return X3D_ERR_UNSUPPORTED;
+  X3D_TRY(ensure_work(c));
+  X3D_TRY(poisson_prepare_f32(c));
+  return capture_graph(c, [&] { return x3d_step_f32(c, gdt, ux, uy, uz, dux, duy, duz, pp3, px, py, pz); });
+}
+
+int x3d_graph_launch(x3d_ctx* c)
+{
+  if (!c) return X3D_ERR_ARG;
+  if (!c->graph_exec) return X3D_ERR_UNSUPPORTED;
+  X3D_CHECK_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+  return X3D_OK;
+}
 
 const char* x3d_pass_name(int i) { return (i >= 0 && i < X3D_NPASS) ? kPass[i].name : nullptr; }
 

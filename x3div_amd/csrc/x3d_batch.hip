@@ -1,0 +1,322 @@
+// The batching form: many independent periodic grids of uneven sizes stepped
+// together.  Every grid is one instance of the reference's problem
+// (src/xcompact3d.f90:80-95 on its own nx, ny, nz); the batch owns their
+// fields, groups the grids by line tiling so that each step pass is one
+// persistent launch per group, and can record the whole batched step in a
+// HIP graph.  An autotuner picks the y/z tiling of each line length by timing
+// the batched step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+
+#include "x3d_internal.h"
+
+using namespace x3d;
+
+namespace {
+
+__global__ void k_to_f32(float* __restrict__ out, const double* __restrict__ in, long long n)
+{
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    out[q] = (float)in[q];
+}
+
+void graph_release(x3d_batch* b)
+{
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+}
+
+int set_streams(x3d_batch* b, hipStream_t s)
+{
+  for (x3d_ctx* c : b->g) {
+    const int rc = x3d_set_stream(c, s);
+    if (rc) return rc;
+  }
+  return X3D_OK;
+}
+
+// One step of every grid: transeq (Euler folded in), divergence, Poisson per
+// grid, gradp with cor_vel.
+int batch_step_body(x3d_batch* b, double gdt)
+{
+  if (!b->built || b->built_gdt != gdt) {
+    const int rc = b->prec == 8 ? batch_build<double>(b, gdt) : batch_build<float>(b, gdt);
+    if (rc) return rc;
+  }
+  auto passes = [b](int which) { return b->prec == 8 ? batch_passes<double>(b, which) : batch_passes<float>(b, which); };
+  int rc = passes(0);
+  if (!rc) rc = passes(1);
+  for (int i = 0; !rc && i < b->ng; ++i) {
+    void* pp = b->field(6, i);
+    rc = b->prec == 8 ? x3d_poisson(b->g[i], static_cast<double*>(pp)) : x3d_poisson_f32(b->g[i], static_cast<float*>(pp));
+  }
+  if (!rc) rc = passes(2);
+  return rc;
+}
+
+// Line lengths that this y/z tiling candidate fits, for the autotuner: the
+// chunk length M, chunk groups per wavefront G, line groups per work-group Wl.
+std::vector<LinePlan> candidates(int n, int prec)
+{
+  static const int ms64[] = {1, 2, 3, 4, 5, 6, 8, 12, 16};
+  static const int ms32[] = {1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32};
+  std::vector<LinePlan> out;
+  const int* ms = prec == 8 ? ms64 : ms32;
+  const int nm = prec == 8 ? 9 : 11;
+  for (int k = 0; k < nm; ++k) {
+    const int m = ms[k];
+    if (n % m) continue;
+    const int c = n / m;
+    if (c > 64) continue;
+    for (int g : {1, 2, 4, 8, 16}) {
+      if (c % g) continue;
+      const int wc = c / g;
+      for (int wl : {1, 2, 4}) {
+        if (wc * wl > 8) continue;
+        LinePlan p;
+        p.n = n; p.M = m; p.C = c; p.G = g; p.Wc = wc; p.Wl = wl; p.LW = 64 / g; p.line_fast = true;
+        out.push_back(p);
+      }
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int x3d_batch_create(int ngrids, const int* dims, const double* lengths, int precision, x3d_batch** out)
+{
+  if (!out || !dims || ngrids <= 0 || (precision != 8 && precision != 4)) return X3D_ERR_ARG;
+  *out = nullptr;
+  x3d_batch* b = new x3d_batch;
+  b->ng = ngrids;
+  b->prec = precision;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  for (int i = 0; i < ngrids; ++i) {
+    const double tau = 2.0 * M_PI;
+    const double lx = lengths ? lengths[3 * i] : tau, ly = lengths ? lengths[3 * i + 1] : tau,
+                 lz = lengths ? lengths[3 * i + 2] : tau;
+    x3d_ctx* c = nullptr;
+    const int rc = x3d_create(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], lx, ly, lz, &c);
+    if (rc) {
+      x3d_batch_destroy(b);
+      return rc;
+    }
+    b->g.push_back(c);
+    b->off.push_back(b->total);
+    b->total += (size_t)dims[3 * i] * dims[3 * i + 1] * dims[3 * i + 2];
+  }
+  if (hipMalloc(&b->store, 10 * b->total * (size_t)precision) != hipSuccess) {
+    x3d_batch_destroy(b);
+    return X3D_ERR_ALLOC;
+  }
+  (void)hipMemset(b->store, 0, 10 * b->total * (size_t)precision);
+  *out = b;
+  return X3D_OK;
+}
+
+int x3d_batch_destroy(x3d_batch* b)
+{
+  if (!b) return X3D_ERR_ARG;
+  graph_release(b);
+  for (x3d_ctx* c : b->g) x3d_destroy(c);
+  if (b->store) (void)hipFree(b->store);
+  if (b->dargs) (void)hipFree(b->dargs);
+  delete b;
+  return X3D_OK;
+}
+
+int x3d_batch_set_stream(x3d_batch* b, void* stream)
+{
+  if (!b) return X3D_ERR_ARG;
+  b->stream = (hipStream_t)stream;
+  return set_streams(b, b->stream);
+}
+
+int x3d_batch_sizes(const x3d_batch* b, int* ngrids, long long* total_points)
+{
+  if (!b) return X3D_ERR_ARG;
+  if (ngrids) *ngrids = b->ng;
+  if (total_points) *total_points = (long long)b->total;
+  return X3D_OK;
+}
+
+int x3d_batch_field(const x3d_batch* b, int grid, int field, void** dptr)
+{
+  if (!b || !dptr || grid < 0 || grid >= b->ng || field < 0 || field >= 10) return X3D_ERR_ARG;
+  *dptr = b->field(field, grid);
+  return X3D_OK;
+}
+
+int x3d_batch_context(const x3d_batch* b, int grid, x3d_ctx** ctx)
+{
+  if (!b || !ctx || grid < 0 || grid >= b->ng) return X3D_ERR_ARG;
+  *ctx = b->g[grid];
+  return X3D_OK;
+}
+
+int x3d_batch_init_fields(x3d_batch* b, int kind, uint64_t seed, double amp)
+{
+  if (!b) return X3D_ERR_ARG;
+  // FP32 batches: each grid's FP64 fields go through one staging buffer
+  double* tmp = nullptr;
+  if (b->prec == 4) {
+    size_t nmax = 0;
+    for (x3d_ctx* c : b->g) nmax = std::max(nmax, (size_t)c->nx * c->ny * c->nz);
+    if (hipMalloc(&tmp, 3 * nmax * sizeof(double)) != hipSuccess) return X3D_ERR_ALLOC;
+  }
+  int rc = X3D_OK;
+  for (int i = 0; i < b->ng && !rc; ++i) {
+    x3d_ctx* c = b->g[i];
+    const size_t n = (size_t)c->nx * c->ny * c->nz;
+    if (b->prec == 8) {
+      rc = x3d_init_fields(c, kind, seed + (uint64_t)i, amp, static_cast<double*>(b->field(0, i)),
+                           static_cast<double*>(b->field(1, i)), static_cast<double*>(b->field(2, i)));
+      continue;
+    }
+    rc = x3d_init_fields(c, kind, seed + (uint64_t)i, amp, tmp, tmp + n, tmp + 2 * n);
+    for (int k = 0; k < 3 && !rc; ++k) {
+      hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, b->stream, static_cast<float*>(b->field(k, i)),
+                         tmp + k * n, (long long)n);
+      if (hipGetLastError() != hipSuccess) rc = X3D_ERR_HIP;
+    }
+  }
+  if (hipStreamSynchronize(b->stream) != hipSuccess && !rc) rc = X3D_ERR_HIP;
+  if (tmp) (void)hipFree(tmp);
+  return rc;
+}
+
+int x3d_batch_step(x3d_batch* b, double gdt)
+{
+  if (!b) return X3D_ERR_ARG;
+  return batch_step_body(b, gdt);
+}
+
+int x3d_batch_graph_capture(x3d_batch* b, double gdt)
+{
+  if (!b) return X3D_ERR_ARG;
+  graph_release(b);
+  // everything the step allocates or uploads happens before the capture
+  int rc = b->prec == 8 ? batch_build<double>(b, gdt) : batch_build<float>(b, gdt);
+  for (int i = 0; !rc && i < b->ng; ++i)
+    if (b->prec == 4) rc = poisson_prepare_f32(b->g[i]);
+  if (rc) return rc;
+  hipStream_t cs = nullptr;
+  X3D_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipStream_t user = b->stream;
+  b->stream = cs;
+  rc = set_streams(b, cs);
+  hipGraph_t g = nullptr;
+  if (!rc) {
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = X3D_ERR_HIP;
+    } else {
+      rc = batch_step_body(b, gdt);
+      const hipError_t e = hipStreamEndCapture(cs, &g);
+      if (!rc && e != hipSuccess) rc = hip_status(e);
+    }
+  }
+  b->stream = user;
+  const int rs = set_streams(b, user);
+  (void)hipStreamDestroy(cs);
+  if (!rc) rc = rs;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  const hipError_t e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    b->graph_exec = nullptr;
+    return hip_status(e);
+  }
+  b->graph = g;
+  return X3D_OK;
+}
+
+int x3d_batch_graph_launch(x3d_batch* b)
+{
+  if (!b) return X3D_ERR_ARG;
+  if (!b->graph_exec) return X3D_ERR_UNSUPPORTED;
+  X3D_CHECK_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+  return X3D_OK;
+}
+
+int x3d_batch_autotune(x3d_batch* b, int reps, char* report, size_t report_len)
+{
+  if (!b || reps <= 0) return X3D_ERR_ARG;
+  graph_release(b);
+  std::string log;
+  auto time_step = [b, reps](double* ms) -> int {
+    int rc = batch_step_body(b, 0.0);  // warm-up, and rebuild after a plan change
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return X3D_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return X3D_ERR_HIP; }
+    (void)hipEventRecord(e0, b->stream);
+    for (int r = 0; r < reps && !rc; ++r) rc = batch_step_body(b, 0.0);
+    (void)hipEventRecord(e1, b->stream);
+    float t = 0.f;
+    if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      rc = X3D_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return rc;
+  };
+  double best_ms = 0.0;
+  int rc = time_step(&best_ms);
+  if (rc) return rc;
+  char line[256];
+  std::snprintf(line, sizeof line, "start %.4f ms/step\n", best_ms);
+  log += line;
+  // coordinate descent: one (direction, line length) at a time, every grid of
+  // that length switched together
+  for (int dir = 1; dir <= 2; ++dir) {
+    std::map<int, std::vector<int>> byn;
+    for (int i = 0; i < b->ng; ++i) byn[dir == 1 ? b->g[i]->ny : b->g[i]->nz].push_back(i);
+    for (auto& kv : byn) {
+      auto plan_of = [&](int i) -> LinePlan& { return b->prec == 8 ? b->g[i]->fplan[dir] : b->g[i]->fplan32[dir]; };
+      const LinePlan keep = plan_of(kv.second[0]);
+      LinePlan best = keep;
+      for (const LinePlan& cand : candidates(kv.first, b->prec)) {
+        if (cand.M == keep.M && cand.G == keep.G && cand.Wc == keep.Wc && cand.Wl == keep.Wl) continue;
+        for (int i : kv.second) plan_of(i) = cand;
+        b->built = false;
+        double ms = 0.0;
+        rc = time_step(&ms);
+        if (rc) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = cand;
+        }
+      }
+      for (int i : kv.second) plan_of(i) = best;
+      b->built = false;
+      if (rc) return rc;
+      std::snprintf(line, sizeof line, "%c n=%d: M=%d G=%d Wc=%d Wl=%d -> %.4f ms/step\n", dir == 1 ? 'y' : 'z',
+                    kv.first, best.M, best.G, best.Wc, best.Wl, best_ms);
+      log += line;
+    }
+  }
+  if (report && report_len) {
+    std::strncpy(report, log.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return X3D_OK;
+}
+
+}  // extern "C"

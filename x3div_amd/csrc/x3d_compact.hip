@@ -21,7 +21,7 @@ Circ<T> make_circ(double alfa, const LinePlan& P)
   Circ<T> c{};
   c.beta = (T)beta;
   double pw = 1.0;
-  for (int k = 0; k <= 16; ++k) {
+  for (int k = 0; k <= 32; ++k) {
     c.pw[k] = (T)(k <= P.M ? pw : 0.0);
     pw *= beta;
   }
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(LF ? 1024 : 256) void k_line(const T* in, T* out, O
   constexpr int HL = Halo<KIND>::L, HR = Halo<KIND>::R;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T>(geo, tl, M);
+  const LineThread t = line_thread<T>(geo, tl, M, blockIdx.x);
   T v[M];
   if constexpr (LF) {
     T win[M + HL + HR];
@@ -154,10 +154,38 @@ int line_plan_fused(int n, bool line_fast, LinePlan* P, int variant)
   return X3D_ERR_UNSUPPORTED;
 }
 
+// FP32 fused y/z passes: the same scheme with chunks up to 32 points, tiled
+// for 128-byte rows where the line allows (32 lines x 2 chunk groups per
+// wavefront), else 16 x 4 or 8 x 8; at most 8 wavefronts per work-group.
+static const int kMs32[] = {1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32};
+
+int line_plan_fused32(int n, bool line_fast, LinePlan* P)
+{
+  if (!line_fast) return line_plan(n, false, P);
+  if (n < 8) return X3D_ERR_SIZE;
+  auto okM = [](int m) { for (int mm : kMs32) if (mm == m) return true; return false; };
+  for (int g : {2, 4, 8}) {
+    for (int c = 8 * g; c >= g; c >>= 1) {
+      if (n % c || !okM(n / c)) continue;
+      LinePlan p;
+      p.n = n; p.line_fast = true; p.C = c; p.M = n / c; p.G = g; p.Wc = c / g; p.LW = 64 / g;
+      p.Wl = p.Wc >= 4 ? 1 : 4 / p.Wc;
+      *P = p;
+      return X3D_OK;
+    }
+  }
+  return X3D_ERR_UNSUPPORTED;
+}
+
 void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, dim3* block, size_t* smem,
                  size_t tbytes, bool fused)
 {
-  const LinePlan& P = fused ? c->fplan[dir] : c->plan[dir];
+  launch_dims_plan(c, dir, fused ? c->fplan[dir] : c->plan[dir], geo, tl, grid, block, smem, tbytes);
+}
+
+void launch_dims_plan(const x3d_ctx* c, int dir, const LinePlan& P, Geom* geo, Tiling* tl, dim3* grid,
+                      dim3* block, size_t* smem, size_t tbytes)
+{
   geo->dir = dir; geo->nx = c->nx; geo->nxy = (long long)c->nx * c->ny; geo->n = P.n;
   geo->nlines = dir == 0 ? c->ny * c->nz : dir == 1 ? c->nx * c->nz : c->nx * c->ny;
   tl->G = P.G; tl->Wc = P.Wc; tl->Wl = P.Wl; tl->LW = P.LW; tl->line_fast = P.line_fast ? 1 : 0;
@@ -190,17 +218,16 @@ static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStre
   return hipGetLastError();
 }
 
-template <bool CIRC>
-static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem, hipStream_t st,
-                             const double* in, double* out, OpDev<double> od, Coef<double> cf,
-                             Circ<double> cc, Geom geo, Tiling tl)
+template <typename T, bool CIRC>
+static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem, hipStream_t st, const T* in,
+                             T* out, OpDev<T> od, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl)
 {
   switch (kind) {
-    case K_D: return launch_kind<double, K_D, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_DVP: return launch_kind<double, K_DVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_DPV: return launch_kind<double, K_DPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_IVP: return launch_kind<double, K_IVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_IPV: return launch_kind<double, K_IPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_D: return launch_kind<T, K_D, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_DVP: return launch_kind<T, K_DVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_DPV: return launch_kind<T, K_DPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_IVP: return launch_kind<T, K_IVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_IPV: return launch_kind<T, K_IPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
     default: return hipErrorInvalidValue;
   }
 }
@@ -232,12 +259,35 @@ int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double*
     cc = make_circ<double>(c->op[op].alfa, P);
     const double ig = 1.0 / circ_gamma(c->op[op].alfa);
     cf = {cf.a * ig, cf.b * ig, cf.c * ig, cf.d * ig};
-    e = launch_any<true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
+    e = launch_any<double, true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   } else if (kind == K_NONE) {
     e = launch_kind<double, K_NONE, false>(P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   } else {
-    e = launch_any<false>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
+    e = launch_any<double, false>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   }
+  return e == hipSuccess ? X3D_OK : hip_status(e);
+}
+
+// FP32 precision form of one operator: the circulant solver in FP32 (the
+// library's periodic tables only; no Thomas-table path).
+int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out)
+{
+  if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS || !in || !out || kind == K_NONE) return X3D_ERR_ARG;
+  const LinePlan& P = c->plan[dir];
+  const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
+  if (c->op[op].n != n || P.n != n) return X3D_ERR_SIZE;
+  if (!c->op[op].circulant) return X3D_ERR_UNSUPPORTED;
+  Geom geo;
+  Tiling tl;
+  dim3 grid, block;
+  size_t smem;
+  launch_dims(c, dir, &geo, &tl, &grid, &block, &smem, sizeof(float));
+  const Coef<double> k = stencil_coef(c->sc[dir], kind);
+  const double ig = 1.0 / circ_gamma(c->op[op].alfa);
+  const Coef<float> cf{(float)(k.a * ig), (float)(k.b * ig), (float)(k.c * ig), (float)(k.d * ig)};
+  const Circ<float> cc = make_circ<float>(c->op[op].alfa, P);
+  const OpDev<float> od{nullptr, nullptr, nullptr, nullptr, (float)c->op[op].alfa};
+  const hipError_t e = launch_any<float, true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   return e == hipSuccess ? X3D_OK : hip_status(e);
 }
 

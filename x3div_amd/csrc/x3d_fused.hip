@@ -8,6 +8,9 @@
 //   gradp     Z -> Y -> X(+cor_vel)                src/navier.f90:249-359, 64-91
 // Each pass reads its input fields once and writes its outputs once; all
 // periodic operators use the circulant line solver.
+#include <cstring>
+#include <map>
+
 #include "x3d_line.h"
 
 namespace x3d {
@@ -16,15 +19,32 @@ namespace x3d {
 // X: du_c = r_c; Y: tmp_c = r_c;
 // Z: du_c = du_c + (tmp_c + r_c); if euler: u_c = gdt du_c + u_c.
 // (src/transeq.f90:100-115, 120-134, 139-153, 158-177 per component)
+template <typename T>
+struct TranseqArgs {
+  T *ux, *uy, *uz, *dux, *duy, *duz, *tx, *ty, *tz;
+  Coef<T> cf;
+  Circ<T> cc;
+  Geom geo;
+  Tiling tl;
+  T gdt;
+  int euler;
+};
+
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) void k_transeq(T* ux, T* uy, T* uz, T* dux, T* duy, T* duz, T* tx,
-                                                  T* ty, T* tz, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl,
-                                                  T gdt, int euler)
+__device__ __forceinline__ void transeq_body(const TranseqArgs<T>& A, int blk)
 {
+  T* const ux = A.ux; T* const uy = A.uy; T* const uz = A.uz;
+  T* const dux = A.dux; T* const duy = A.duy; T* const duz = A.duz;
+  T* const tx = A.tx; T* const ty = A.ty; T* const tz = A.tz;
+  const Coef<T> cf = A.cf;
+  const Circ<T>& cc = A.cc;
+  const Geom geo = A.geo;
+  const T gdt = A.gdt;
+  const int euler = A.euler;
   constexpr int H = 2, NW = M + 2 * H;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T, DIR>(geo, tl, M);
+  const LineThread t = line_thread<T, DIR>(geo, A.tl, M, blk);
   const int n = geo.n;
   T* const Ud = DIR == 0 ? ux : (DIR == 1 ? uy : uz);
   XBlock<T, M> xb;
@@ -95,14 +115,33 @@ __global__ __launch_bounds__(512) void k_transeq(T* ux, T* uy, T* uz, T* dux, T*
 // X: o1 = Dvp(a), o2 = Ivp(b), o3 = Ivp(c)        a, b, c = ux, uy, uz
 // Y: o1 = Dvp(b) + Ivp(a), o2 = Ivp(c)             a, b, c = pp1, pgy1, pgz1
 // Z: o1 = Ivp(a) + Dvp(b)                          a, b = duydypi2, upi2
+// Arguments of the divergence and gradient passes.
+template <typename T>
+struct PassArgs {
+  const T *a, *b, *c;
+  T *o1, *o2, *o3;
+  T *ux, *uy, *uz;
+  Coef<T> cfd;
+  Circ<T> ccd;
+  Coef<T> cfi;
+  Circ<T> cci;
+  Geom geo;
+  Tiling tl;
+  int corr;
+};
+
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) void k_div(const T* a, const T* b, const T* c, T* o1, T* o2, T* o3,
-                                              Coef<T> cfd, Circ<T> ccd, Coef<T> cfi, Circ<T> cci, Geom geo,
-                                              Tiling tl)
+__device__ __forceinline__ void div_body(const PassArgs<T>& A, int blk)
 {
+  const T* const a = A.a; const T* const b = A.b; const T* const c = A.c;
+  T* const o1 = A.o1; T* const o2 = A.o2; T* const o3 = A.o3;
+  const Coef<T> cfd = A.cfd, cfi = A.cfi;
+  const Circ<T>& ccd = A.ccd;
+  const Circ<T>& cci = A.cci;
+  const Geom geo = A.geo;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T, DIR>(geo, tl, M);
+  const LineThread t = line_thread<T, DIR>(geo, A.tl, M, blk);
   const int n = geo.n;
   if constexpr (DIR == 0) {
     const XBlock<T, M> xb = x_block<T, M>(geo, t, lds);
@@ -178,13 +217,19 @@ __global__ __launch_bounds__(512) void k_div(const T* a, const T* b, const T* c,
 // X: o1 = Dpv(a), o2 = Ipv(b), o3 = Ipv(c)          a, b, c = ppi2, pgy2, pgzi2
 //    if corr: u_k = u_k - o_k (src/navier.f90:84-86)
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) void k_grad(const T* a, const T* b, const T* c, T* o1, T* o2, T* o3,
-                                               T* ux, T* uy, T* uz, Coef<T> cfd, Circ<T> ccd, Coef<T> cfi,
-                                               Circ<T> cci, Geom geo, Tiling tl, int corr)
+__device__ __forceinline__ void grad_body(const PassArgs<T>& A, int blk)
 {
+  const T* const a = A.a; const T* const b = A.b; const T* const c = A.c;
+  T* const o1 = A.o1; T* const o2 = A.o2; T* const o3 = A.o3;
+  T* const ux = A.ux; T* const uy = A.uy; T* const uz = A.uz;
+  const Coef<T> cfd = A.cfd, cfi = A.cfi;
+  const Circ<T>& ccd = A.ccd;
+  const Circ<T>& cci = A.cci;
+  const Geom geo = A.geo;
+  const int corr = A.corr;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T, DIR>(geo, tl, M);
+  const LineThread t = line_thread<T, DIR>(geo, A.tl, M, blk);
   const int n = geo.n;
   if constexpr (DIR == 2 || DIR == 1) {
     T v[M];
@@ -234,89 +279,169 @@ __global__ __launch_bounds__(512) void k_grad(const T* a, const T* b, const T* c
   }
 }
 
+// ---- kernels: one grid (blockIdx.x is the line block) or many grids in one
+// persistent launch (each work-group walks virtual blocks; block0[g] is the
+// first virtual block of grid g, block0[ngrids] the total) -----------------
+__device__ __forceinline__ int grid_of(const int* __restrict__ block0, int ngrids, int vb)
+{
+  int lo = 0, hi = ngrids - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (block0[mid] <= vb) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, int M, int DIR>
+__global__ __launch_bounds__(512) void k_transeq(TranseqArgs<T> A) { transeq_body<T, M, DIR>(A, blockIdx.x); }
+template <typename T, int M, int DIR>
+__global__ __launch_bounds__(512) void k_div(PassArgs<T> A) { div_body<T, M, DIR>(A, blockIdx.x); }
+template <typename T, int M, int DIR>
+__global__ __launch_bounds__(512) void k_grad(PassArgs<T> A) { grad_body<T, M, DIR>(A, blockIdx.x); }
+
+template <typename T, int M, int DIR>
+__global__ __launch_bounds__(512) void k_transeq_batch(const TranseqArgs<T>* __restrict__ list,
+                                                       const int* __restrict__ block0, int ngrids)
+{
+  const int total = block0[ngrids];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int g = grid_of(block0, ngrids, vb);
+    transeq_body<T, M, DIR>(list[g], vb - block0[g]);
+  }
+}
+template <typename T, int M, int DIR>
+__global__ __launch_bounds__(512) void k_div_batch(const PassArgs<T>* __restrict__ list,
+                                                   const int* __restrict__ block0, int ngrids)
+{
+  const int total = block0[ngrids];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int g = grid_of(block0, ngrids, vb);
+    div_body<T, M, DIR>(list[g], vb - block0[g]);
+  }
+}
+template <typename T, int M, int DIR>
+__global__ __launch_bounds__(512) void k_grad_batch(const PassArgs<T>* __restrict__ list,
+                                                    const int* __restrict__ block0, int ngrids)
+{
+  const int total = block0[ngrids];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int g = grid_of(block0, ngrids, vb);
+    grad_body<T, M, DIR>(list[g], vb - block0[g]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+template <typename T>
 struct DirSet {
   Geom geo;
   Tiling tl;
   dim3 grid, block;
   size_t smem;
   int M;
-  Coef<double> cf_d, cf_dvp, cf_dpv, cf_ivp, cf_ipv;  // scaled by 1/gamma
-  Circ<double> cc_d, cc_s, cc_i;
+  Coef<T> cf_d, cf_dvp, cf_dpv, cf_ivp, cf_ipv;  // scaled by 1/gamma
+  Circ<T> cc_d, cc_s, cc_i;
 };
 
-static DirSet dirset(const x3d_ctx* c, int dir)
+// Plans: FP64 uses the fused plans, FP32 its own (longer chunks, 128-byte rows).
+template <typename T>
+static const LinePlan& fused_plan(const x3d_ctx* c, int dir)
 {
-  DirSet s;
-  launch_dims(c, dir, &s.geo, &s.tl, &s.grid, &s.block, &s.smem, sizeof(double), true);
-  const LinePlan& P = c->fplan[dir];
+  return sizeof(T) == 8 ? c->fplan[dir] : c->fplan32[dir];
+}
+
+template <typename T>
+static DirSet<T> dirset(const x3d_ctx* c, int dir)
+{
+  DirSet<T> s;
+  const LinePlan& P = fused_plan<T>(c, dir);
+  launch_dims_plan(c, dir, P, &s.geo, &s.tl, &s.grid, &s.block, &s.smem, sizeof(T));
   s.M = P.M;
   const Scheme& sc = c->sc[dir];
+  // coefficients in FP64, rounded once to T
   auto scaled = [](Coef<double> k, double alfa) {
     const double ig = 1.0 / circ_gamma(alfa);
-    return Coef<double>{k.a * ig, k.b * ig, k.c * ig, k.d * ig};
+    return Coef<T>{(T)(k.a * ig), (T)(k.b * ig), (T)(k.c * ig), (T)(k.d * ig)};
   };
   s.cf_d = scaled(stencil_coef(sc, K_D), sc.alfai);
   s.cf_dvp = scaled(stencil_coef(sc, K_DVP), sc.alcai);
   s.cf_dpv = scaled(stencil_coef(sc, K_DPV), sc.alcai);
   s.cf_ivp = scaled(stencil_coef(sc, K_IVP), sc.ailcai);
   s.cf_ipv = scaled(stencil_coef(sc, K_IPV), sc.ailcai);
-  s.cc_d = make_circ<double>(sc.alfai, P);
-  s.cc_s = make_circ<double>(sc.alcai, P);
-  s.cc_i = make_circ<double>(sc.ailcai, P);
+  s.cc_d = make_circ<T>(sc.alfai, P);
+  s.cc_s = make_circ<T>(sc.alcai, P);
+  s.cc_i = make_circ<T>(sc.ailcai, P);
   return s;
 }
 
-#define X3D_MSWITCH(M, CALL)       \
-  switch (M) {                     \
-    case 1: { constexpr int MM = 1; CALL; } break;   \
-    case 2: { constexpr int MM = 2; CALL; } break;   \
-    case 3: { constexpr int MM = 3; CALL; } break;   \
-    case 4: { constexpr int MM = 4; CALL; } break;   \
-    case 5: { constexpr int MM = 5; CALL; } break;   \
-    case 6: { constexpr int MM = 6; CALL; } break;   \
-    case 8: { constexpr int MM = 8; CALL; } break;   \
-    case 12: { constexpr int MM = 12; CALL; } break; \
-    case 16: { constexpr int MM = 16; CALL; } break; \
-    default: return X3D_ERR_UNSUPPORTED;             \
-  }
+// Chunk lengths instantiated per element type: FP64 up to 16 points, FP32 up
+// to 32 (the same registers per thread).
+#define X3D_MSWITCH_T(T, M, CALL_)                                                          \
+  do {                                                                                      \
+    if constexpr (sizeof(T) == 8) {                                                         \
+      switch (M) {                                                                          \
+        case 1: { constexpr int MM = 1; CALL_; } break;                          \
+        case 2: { constexpr int MM = 2; CALL_; } break;                          \
+        case 3: { constexpr int MM = 3; CALL_; } break;                          \
+        case 4: { constexpr int MM = 4; CALL_; } break;                          \
+        case 5: { constexpr int MM = 5; CALL_; } break;                          \
+        case 6: { constexpr int MM = 6; CALL_; } break;                          \
+        case 8: { constexpr int MM = 8; CALL_; } break;                          \
+        case 12: { constexpr int MM = 12; CALL_; } break;                          \
+        case 16: { constexpr int MM = 16; CALL_; } break;                          \
+        default: return X3D_ERR_UNSUPPORTED;                                                \
+      }                                                                                     \
+    } else {                                                                                \
+      switch (M) {                                                                          \
+        case 1: { constexpr int MM = 1; CALL_; } break;                          \
+        case 2: { constexpr int MM = 2; CALL_; } break;                          \
+        case 3: { constexpr int MM = 3; CALL_; } break;                          \
+        case 4: { constexpr int MM = 4; CALL_; } break;                          \
+        case 5: { constexpr int MM = 5; CALL_; } break;                          \
+        case 6: { constexpr int MM = 6; CALL_; } break;                          \
+        case 8: { constexpr int MM = 8; CALL_; } break;                          \
+        case 12: { constexpr int MM = 12; CALL_; } break;                          \
+        case 16: { constexpr int MM = 16; CALL_; } break;                          \
+        case 24: { constexpr int MM = 24; CALL_; } break;                          \
+        case 32: { constexpr int MM = 32; CALL_; } break;                          \
+        default: return X3D_ERR_UNSUPPORTED;                                                \
+      }                                                                                     \
+    }                                                                                       \
+  } while (0)
 
-template <int DIR>
-static int transeq_pass(x3d_ctx* c, double* ux, double* uy, double* uz, double* dux, double* duy, double* duz,
-                        double* tx, double* ty, double* tz, double gdt, int euler)
+template <int DIR, typename T>
+static int transeq_pass(x3d_ctx* c, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz, T* tx, T* ty, T* tz, T gdt,
+                        int euler)
 {
-  const DirSet s = dirset(c, DIR);
-  X3D_MSWITCH(s.M, hipLaunchKernelGGL((k_transeq<double, MM, DIR>), s.grid, s.block, s.smem, c->stream, ux, uy,
-                                      uz, dux, duy, duz, tx, ty, tz, s.cf_d, s.cc_d, s.geo, s.tl, gdt, euler));
+  const DirSet<T> s = dirset<T>(c, DIR);
+  const TranseqArgs<T> A{ux, uy, uz, dux, duy, duz, tx, ty, tz, s.cf_d, s.cc_d, s.geo, s.tl, gdt, euler};
+  X3D_MSWITCH_T(T, s.M, hipLaunchKernelGGL((k_transeq<T, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
 
-template <int DIR>
-static int div_pass(x3d_ctx* c, const double* a, const double* b, const double* cc, double* o1, double* o2,
-                    double* o3)
+template <int DIR, typename T>
+static int div_pass(x3d_ctx* c, const T* a, const T* b, const T* cc, T* o1, T* o2, T* o3)
 {
-  const DirSet s = dirset(c, DIR);
-  X3D_MSWITCH(s.M, hipLaunchKernelGGL((k_div<double, MM, DIR>), s.grid, s.block, s.smem, c->stream, a, b, cc,
-                                      o1, o2, o3, s.cf_dvp, s.cc_s, s.cf_ivp, s.cc_i, s.geo, s.tl));
+  const DirSet<T> s = dirset<T>(c, DIR);
+  const PassArgs<T> A{a, b, cc, o1, o2, o3, nullptr, nullptr, nullptr, s.cf_dvp, s.cc_s, s.cf_ivp, s.cc_i,
+                      s.geo, s.tl, 0};
+  X3D_MSWITCH_T(T, s.M, hipLaunchKernelGGL((k_div<T, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
 
-template <int DIR>
-static int grad_pass(x3d_ctx* c, const double* a, const double* b, const double* cc, double* o1, double* o2,
-                     double* o3, double* ux, double* uy, double* uz, int corr)
+template <int DIR, typename T>
+static int grad_pass(x3d_ctx* c, const T* a, const T* b, const T* cc, T* o1, T* o2, T* o3, T* ux, T* uy, T* uz,
+                     int corr)
 {
-  const DirSet s = dirset(c, DIR);
-  X3D_MSWITCH(s.M, hipLaunchKernelGGL((k_grad<double, MM, DIR>), s.grid, s.block, s.smem, c->stream, a, b, cc,
-                                      o1, o2, o3, ux, uy, uz, s.cf_dpv, s.cc_s, s.cf_ipv, s.cc_i, s.geo, s.tl,
-                                      corr));
+  const DirSet<T> s = dirset<T>(c, DIR);
+  const PassArgs<T> A{a, b, cc, o1, o2, o3, ux, uy, uz, s.cf_dpv, s.cc_s, s.cf_ipv, s.cc_i, s.geo, s.tl, corr};
+  X3D_MSWITCH_T(T, s.M, hipLaunchKernelGGL((k_grad<T, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
-#undef X3D_MSWITCH
 
 bool fused_ok(const x3d_ctx* c)
 {
@@ -327,45 +452,193 @@ bool fused_ok(const x3d_ctx* c)
 
 #define X3D_TRY(expr) do { int _rc = (expr); if (_rc) return _rc; } while (0)
 
-int fused_momentum(x3d_ctx* c, double gdt, int euler, double* ux, double* uy, double* uz, double* dux,
-                   double* duy, double* duz)
+// Scratch fields: the FP64 work arrays, viewed as T (FP32 uses half of each).
+template <typename T>
+static T* wk(x3d_ctx* c, int i) { return reinterpret_cast<T*>(c->work[i]); }
+
+template <typename T>
+int fused_momentum(x3d_ctx* c, T gdt, int euler, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz)
 {
   X3D_TRY(ensure_work(c));
-  double *tx = c->work[5], *ty = c->work[6], *tz = c->work[7];
-  X3D_TRY(transeq_pass<0>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0));
+  T *tx = wk<T>(c, 5), *ty = wk<T>(c, 6), *tz = wk<T>(c, 7);
+  X3D_TRY((transeq_pass<0, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0)));
   X3D_TRY(mark_pass(c, 1));
-  X3D_TRY(transeq_pass<1>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0));
+  X3D_TRY((transeq_pass<1, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0)));
   X3D_TRY(mark_pass(c, 2));
-  X3D_TRY(transeq_pass<2>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, euler));
+  X3D_TRY((transeq_pass<2, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, euler)));
   X3D_TRY(mark_pass(c, 3));
   return X3D_OK;
 }
 
-int fused_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz)
+template <typename T>
+int fused_divergence(x3d_ctx* c, T* pp3, const T* ux, const T* uy, const T* uz)
 {
   X3D_TRY(ensure_work(c));
-  double *pp1 = c->work[0], *pgy1 = c->work[1], *pgz1 = c->work[2], *dd2 = c->work[3], *upi2 = c->work[4];
-  X3D_TRY(div_pass<0>(c, ux, uy, uz, pp1, pgy1, pgz1));
+  T *pp1 = wk<T>(c, 0), *pgy1 = wk<T>(c, 1), *pgz1 = wk<T>(c, 2), *dd2 = wk<T>(c, 3), *upi2 = wk<T>(c, 4);
+  X3D_TRY((div_pass<0, T>(c, ux, uy, uz, pp1, pgy1, pgz1)));
   X3D_TRY(mark_pass(c, 4));
-  X3D_TRY(div_pass<1>(c, pp1, pgy1, pgz1, dd2, upi2, nullptr));
+  X3D_TRY((div_pass<1, T>(c, pp1, pgy1, pgz1, dd2, upi2, nullptr)));
   X3D_TRY(mark_pass(c, 5));
-  X3D_TRY(div_pass<2>(c, dd2, upi2, nullptr, pp3, nullptr, nullptr));
+  X3D_TRY((div_pass<2, T>(c, dd2, upi2, nullptr, pp3, nullptr, nullptr)));
   X3D_TRY(mark_pass(c, 6));
   return X3D_OK;
 }
 
-int fused_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3, double* ux, double* uy,
-                double* uz, int corr)
+template <typename T>
+int fused_gradp(x3d_ctx* c, T* px, T* py, T* pz, const T* pp3, T* ux, T* uy, T* uz, int corr)
 {
   X3D_TRY(ensure_work(c));
-  double *ppi3 = c->work[0], *pgz3 = c->work[1], *ppi2 = c->work[2], *pgy2 = c->work[3], *pgzi2 = c->work[4];
-  X3D_TRY(grad_pass<2>(c, pp3, nullptr, nullptr, ppi3, pgz3, nullptr, nullptr, nullptr, nullptr, 0));
+  T *ppi3 = wk<T>(c, 0), *pgz3 = wk<T>(c, 1), *ppi2 = wk<T>(c, 2), *pgy2 = wk<T>(c, 3), *pgzi2 = wk<T>(c, 4);
+  X3D_TRY((grad_pass<2, T>(c, pp3, nullptr, nullptr, ppi3, pgz3, nullptr, nullptr, nullptr, nullptr, 0)));
   X3D_TRY(mark_pass(c, 8));
-  X3D_TRY(grad_pass<1>(c, ppi3, pgz3, nullptr, ppi2, pgy2, pgzi2, nullptr, nullptr, nullptr, 0));
+  X3D_TRY((grad_pass<1, T>(c, ppi3, pgz3, nullptr, ppi2, pgy2, pgzi2, nullptr, nullptr, nullptr, 0)));
   X3D_TRY(mark_pass(c, 9));
-  X3D_TRY(grad_pass<0>(c, ppi2, pgy2, pgzi2, px, py, pz, ux, uy, uz, corr));
+  X3D_TRY((grad_pass<0, T>(c, ppi2, pgy2, pgzi2, px, py, pz, ux, uy, uz, corr)));
   X3D_TRY(mark_pass(c, 10));
   return X3D_OK;
 }
+
+// ---- batched passes ----------------------------------------------------------
+namespace {
+struct PlanKey {
+  int M, G, Wc, Wl, LW;
+  bool operator<(const PlanKey& o) const
+  {
+    return M != o.M ? M < o.M : G != o.G ? G < o.G : Wc != o.Wc ? Wc < o.Wc : Wl != o.Wl ? Wl < o.Wl : LW < o.LW;
+  }
+};
+constexpr int kPassDir[9] = {0, 1, 2, 0, 1, 2, 2, 1, 0};
+}  // namespace
+
+template <typename T>
+int batch_build(x3d_batch* b, double gdt)
+{
+  std::vector<unsigned char> host;
+  auto put = [&host](const void* p, size_t n) {
+    size_t at = (host.size() + 255) & ~size_t(255);
+    host.resize(at + n);
+    std::memcpy(host.data() + at, p, n);
+    return at;
+  };
+  b->launches.clear();
+  for (int i = 0; i < b->ng; ++i) X3D_TRY(ensure_work(b->g[i]));
+  for (int pass = 0; pass < 9; ++pass) {
+    const int dir = kPassDir[pass];
+    std::map<PlanKey, std::vector<int>> groups;
+    for (int i = 0; i < b->ng; ++i) {
+      const LinePlan& P = fused_plan<T>(b->g[i], dir);
+      if (!P.M) return X3D_ERR_UNSUPPORTED;
+      groups[PlanKey{P.M, P.G, P.Wc, P.Wl, P.LW}].push_back(i);
+    }
+    for (auto& kv : groups) {
+      std::vector<int> block0{0};
+      x3d_batch::Launch L{};
+      L.pass = pass;
+      L.M = kv.first.M;
+      std::vector<TranseqArgs<T>> ta;
+      std::vector<PassArgs<T>> pa;
+      for (int i : kv.second) {
+        x3d_ctx* c = b->g[i];
+        const DirSet<T> s = dirset<T>(c, dir);
+        T* f[10];
+        for (int k = 0; k < 10; ++k) f[k] = static_cast<T*>(b->field(k, i));
+        T* w[8];
+        for (int k = 0; k < 8; ++k) w[k] = wk<T>(c, k);
+        if (pass < 3) {
+          ta.push_back(TranseqArgs<T>{f[0], f[1], f[2], f[3], f[4], f[5], w[5], w[6], w[7], s.cf_d, s.cc_d, s.geo,
+                                      s.tl, (T)gdt, pass == 2 ? 1 : 0});
+        } else if (pass < 6) {
+          const T *a = nullptr, *bb = nullptr, *cc = nullptr;
+          T *o1 = nullptr, *o2 = nullptr, *o3 = nullptr;
+          if (pass == 3) { a = f[0]; bb = f[1]; cc = f[2]; o1 = w[0]; o2 = w[1]; o3 = w[2]; }
+          if (pass == 4) { a = w[0]; bb = w[1]; cc = w[2]; o1 = w[3]; o2 = w[4]; }
+          if (pass == 5) { a = w[3]; bb = w[4]; o1 = f[6]; }
+          pa.push_back(PassArgs<T>{a, bb, cc, o1, o2, o3, nullptr, nullptr, nullptr, s.cf_dvp, s.cc_s, s.cf_ivp,
+                                   s.cc_i, s.geo, s.tl, 0});
+        } else {
+          const T *a = nullptr, *bb = nullptr, *cc = nullptr;
+          T *o1 = nullptr, *o2 = nullptr, *o3 = nullptr, *u1 = nullptr, *u2 = nullptr, *u3 = nullptr;
+          if (pass == 6) { a = f[6]; o1 = w[0]; o2 = w[1]; }
+          if (pass == 7) { a = w[0]; bb = w[1]; o1 = w[2]; o2 = w[3]; o3 = w[4]; }
+          if (pass == 8) { a = w[2]; bb = w[3]; cc = w[4]; o1 = f[7]; o2 = f[8]; o3 = f[9]; u1 = f[0]; u2 = f[1]; u3 = f[2]; }
+          pa.push_back(PassArgs<T>{a, bb, cc, o1, o2, o3, u1, u2, u3, s.cf_dpv, s.cc_s, s.cf_ipv, s.cc_i, s.geo,
+                                   s.tl, pass == 8 ? 1 : 0});
+        }
+        block0.push_back(block0.back() + (int)s.grid.x);
+        L.block = s.block;
+        L.smem = s.smem;
+      }
+      L.ngroup = (int)kv.second.size();
+      L.args = pass < 3 ? put(ta.data(), ta.size() * sizeof(TranseqArgs<T>))
+                        : put(pa.data(), pa.size() * sizeof(PassArgs<T>));
+      L.block0 = put(block0.data(), block0.size() * sizeof(int));
+      // persistent: at most a few work-groups per compute unit walk the blocks
+      const int cap = 4 * b->ncu;
+      L.grid = dim3((unsigned)(block0.back() < cap ? block0.back() : cap));
+      b->launches.push_back(L);
+    }
+  }
+  if (host.size() > b->dargs_bytes) {
+    if (b->dargs) (void)hipFree(b->dargs);
+    b->dargs = nullptr;
+    b->dargs_bytes = 0;
+    X3D_CHECK_HIP(hipMalloc(&b->dargs, host.size()));
+    b->dargs_bytes = host.size();
+  }
+  X3D_CHECK_HIP(hipMemcpy(b->dargs, host.data(), host.size(), hipMemcpyHostToDevice));
+  b->built = true;
+  b->built_gdt = gdt;
+  return X3D_OK;
+}
+
+template <typename T>
+static int batch_launch(x3d_batch* b, const x3d_batch::Launch& L)
+{
+  const char* base = static_cast<const char*>(b->dargs);
+  const int* block0 = reinterpret_cast<const int*>(base + L.block0);
+  const void* args = base + L.args;
+  const TranseqArgs<T>* ta = static_cast<const TranseqArgs<T>*>(args);
+  const PassArgs<T>* pa = static_cast<const PassArgs<T>*>(args);
+  hipStream_t st = b->stream;
+  switch (L.pass) {
+#define X3D_BT(P, K, DIR, LIST)                                                                             \
+  case P:                                                                                                 \
+    X3D_MSWITCH_T(T, L.M, hipLaunchKernelGGL((K<T, MM, DIR>), L.grid, L.block, L.smem, st, LIST, block0, \
+                                             L.ngroup));                                                  \
+    break;
+    X3D_BT(0, k_transeq_batch, 0, ta) X3D_BT(1, k_transeq_batch, 1, ta) X3D_BT(2, k_transeq_batch, 2, ta)
+    X3D_BT(3, k_div_batch, 0, pa) X3D_BT(4, k_div_batch, 1, pa) X3D_BT(5, k_div_batch, 2, pa)
+    X3D_BT(6, k_grad_batch, 2, pa) X3D_BT(7, k_grad_batch, 1, pa) X3D_BT(8, k_grad_batch, 0, pa)
+#undef X3D_BT
+    default: return X3D_ERR_ARG;
+  }
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+template <typename T>
+int batch_passes(x3d_batch* b, int which)
+{
+  if (!b->built) return X3D_ERR_ARG;
+  for (int p = 3 * which; p < 3 * which + 3; ++p)
+    for (const auto& L : b->launches)
+      if (L.pass == p) X3D_TRY(batch_launch<T>(b, L));
+  return X3D_OK;
+}
+
+template int batch_build<double>(x3d_batch*, double);
+template int batch_build<float>(x3d_batch*, double);
+template int batch_passes<double>(x3d_batch*, int);
+template int batch_passes<float>(x3d_batch*, int);
+
+template int fused_momentum<double>(x3d_ctx*, double, int, double*, double*, double*, double*, double*, double*);
+template int fused_momentum<float>(x3d_ctx*, float, int, float*, float*, float*, float*, float*, float*);
+template int fused_divergence<double>(x3d_ctx*, double*, const double*, const double*, const double*);
+template int fused_divergence<float>(x3d_ctx*, float*, const float*, const float*, const float*);
+template int fused_gradp<double>(x3d_ctx*, double*, double*, double*, const double*, double*, double*, double*,
+                                 int);
+template int fused_gradp<float>(x3d_ctx*, float*, float*, float*, const float*, float*, float*, float*, int);
+
+#undef X3D_MSWITCH_T
 
 }  // namespace x3d

@@ -75,9 +75,12 @@ struct x3d_ctx {
   size_t tab_len = 0;
   x3d::LinePlan plan[3];   // single-operator kernels
   x3d::LinePlan fplan[3];  // fused step passes (at most 512 threads per work-group)
+  x3d::LinePlan fplan32[3];  // fused step passes of the FP32 precision form
   // Poisson
   hipfftHandle fft_r2c = 0, fft_c2r = 0;
   bool fft_ready = false;
+  hipfftHandle fft_r2c32 = 0, fft_c2r32 = 0;  // FP32 plans, made on first use
+  bool fft32_ready = false;
   double2* spec = nullptr;          // (nx/2+1) * ny * nz complex
   double* ktab = nullptr;           // device 1-D tables, see poisson source
   std::vector<double> ax, bx, ay, by, az, bz;
@@ -89,15 +92,62 @@ struct x3d_ctx {
   // per-pass timing of x3d_step_timed: event i is recorded after pass i
   hipEvent_t pass_ev[X3D_NPASS + 1] = {};
   bool timing = false;
+  // captured step (x3d_graph_capture_step*), replayed by x3d_graph_launch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+// A batch of independent grids (the batching form).  Each grid keeps its own
+// context (plans, operator tables, FFT plans, scratch fields); the batch owns
+// the fields of all grids, 10 blocks (ux uy uz dux duy duz pp3 px py pz) of
+// all grids' points, grid after grid, and launches each step pass once per
+// group of grids that share a line tiling.
+struct x3d_batch {
+  int ng = 0;
+  int prec = 8;  // bytes per value: 8 FP64, 4 FP32
+  std::vector<x3d_ctx*> g;
+  std::vector<size_t> off;  // first point of each grid inside a field block
+  size_t total = 0;         // points of all grids
+  void* store = nullptr;
+  hipStream_t stream = nullptr;
+  int ncu = 256;
+  // device argument records of the grouped launches, rebuilt when the plans
+  // or gdt change
+  struct Launch {
+    int pass;       // 0-2 transeq x y z, 3-5 divergence x y z, 6-8 gradp z y x
+    int M;
+    dim3 grid, block;
+    size_t smem;
+    size_t args;    // byte offset of the group's records in dargs
+    size_t block0;  // byte offset of its block prefix (ngroup + 1 ints)
+    int ngroup;
+  };
+  std::vector<Launch> launches;
+  void* dargs = nullptr;
+  size_t dargs_bytes = 0;
+  bool built = false;
+  double built_gdt = 0.0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  void* field(int f, int i) const { return (char*)store + ((size_t)f * total + off[i]) * (size_t)prec; }
 };
 
 namespace x3d {
 
 int hip_status(hipError_t e);
+
+// Batched step passes (x3d_fused.hip): build the grouped launch records, and
+// run the three directional passes of transeq (which = 0), divergence (1)
+// or gradp with cor_vel (2) for every grid of the batch.
+template <typename T>
+int batch_build(x3d_batch* b, double gdt);
+template <typename T>
+int batch_passes(x3d_batch* b, int which);
 int line_plan(int n, bool line_fast, LinePlan* out);
 int line_plan_fused(int n, bool line_fast, LinePlan* out, int variant);
 
 // Launch one compact operator (or Thomas alone) along direction dir.
+int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out);
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
                 bool periodic_fix);
 
@@ -105,6 +155,9 @@ struct Geom;
 struct Tiling;
 void launch_dims(const x3d_ctx* c, int dir, Geom* geo, Tiling* tl, dim3* grid, dim3* block, size_t* smem,
                  size_t tbytes, bool fused = false);
+void launch_dims_plan(const x3d_ctx* c, int dir, const LinePlan& P, Geom* geo, Tiling* tl, dim3* grid,
+                      dim3* block, size_t* smem, size_t tbytes);
+int line_plan_fused32(int n, bool line_fast, LinePlan* out);
 
 int poisson_setup(x3d_ctx* c);
 void poisson_release(x3d_ctx* c);
@@ -121,11 +174,16 @@ inline int mark_pass(x3d_ctx* c, int i)
 
 // Fused step passes (all operators must carry the library's periodic tables).
 bool fused_ok(const x3d_ctx* c);
-int fused_momentum(x3d_ctx* c, double gdt, int euler, double* ux, double* uy, double* uz, double* dux,
-                   double* duy, double* duz);
-int fused_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz);
-int fused_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3, double* ux, double* uy,
-                double* uz, int corr);
+template <typename T>
+int fused_momentum(x3d_ctx* c, T gdt, int euler, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz);
+template <typename T>
+int fused_divergence(x3d_ctx* c, T* pp3, const T* ux, const T* uy, const T* uz);
+template <typename T>
+int fused_gradp(x3d_ctx* c, T* px, T* py, T* pz, const T* pp3, T* ux, T* uy, T* uz, int corr);
+
+// FP32 Poisson: FP32 FFTs around an FP64 spectral division.
+int poisson_prepare_f32(x3d_ctx* c);
+int poisson_solve_f32(x3d_ctx* c, float* rhs);
 
 }  // namespace x3d
 

@@ -80,7 +80,7 @@ struct Circ {
   T close;       // 1 / (1 - beta^n)
   T pwMG;        // beta^(M G)
   T pwM2[7];     // beta^(M 2^k)
-  T pw[17];      // beta^k, k = 0..M
+  T pw[33];      // beta^k, k = 0..M (M <= 32)
 };
 
 struct LineThread {
@@ -121,8 +121,10 @@ __device__ __forceinline__ void st(T* base, uint32_t boff, T v)
 }
 
 // DIRC >= 0 fixes the direction at compile time; otherwise geo.dir selects it.
+// blk: the work-group's block of lines (blockIdx.x, or a virtual block when
+// one launch serves several grids).
 template <typename T, int DIRC = -1>
-__device__ __forceinline__ LineThread line_thread(const Geom& geo, const Tiling& tl, int M)
+__device__ __forceinline__ LineThread line_thread(const Geom& geo, const Tiling& tl, int M, int blk)
 {
   LineThread t;
   t.G = tl.G; t.Wc = tl.Wc; t.LW = tl.LW; t.lf = tl.line_fast;
@@ -133,7 +135,7 @@ __device__ __forceinline__ LineThread line_thread(const Geom& geo, const Tiling&
   else { t.lineLocal = lane / tl.G; t.g = lane - t.lineLocal * tl.G; }
   t.LPW = tl.Wl * tl.LW;
   t.ls = wl * tl.LW + t.lineLocal;
-  t.line = blockIdx.x * t.LPW + t.ls;
+  t.line = blk * t.LPW + t.ls;
   t.active = t.line < geo.nlines;
   t.C = tl.G * tl.Wc;
   t.c = t.wc * tl.G + t.g;
@@ -247,7 +249,7 @@ __device__ __forceinline__ XBlock<T, M> x_block(const Geom& geo, const LineThrea
   const int wave = threadIdx.x >> 6;
   b.lane = threadIdx.x & 63;
   b.tile = tiles + wave * x_tile_words<M>();
-  const int first = blockIdx.x * t.LPW + (t.ls - t.lineLocal);  // first line of this wavefront
+  const int first = t.line - t.lineLocal;  // first line of this wavefront
   const long long left = (long long)(geo.nlines - first) * geo.n;
   b.valid = left <= 0 ? 0 : (left >= 64 * M ? 64 * M : (int)left);
   b.base = (uint32_t)first * (uint32_t)geo.n * (uint32_t)sizeof(T);

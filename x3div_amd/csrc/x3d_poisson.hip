@@ -118,10 +118,84 @@ void poisson_release(x3d_ctx* c)
     hipfftDestroy(c->fft_c2r);
     c->fft_ready = false;
   }
+  if (c->fft32_ready) {
+    hipfftDestroy(c->fft_r2c32);
+    hipfftDestroy(c->fft_c2r32);
+    c->fft32_ready = false;
+  }
   if (c->spec) (void)hipFree(c->spec);
   if (c->ktab) (void)hipFree(c->ktab);
   c->spec = nullptr;
   c->ktab = nullptr;
+}
+
+// FP32 spectrum, FP64 division: the modified wavenumber and the quotient are
+// formed in FP64 (the division near K = 0 is where FP32 would lose digits) and
+// rounded once to FP32.
+__global__ void k_spectral_f32(float2* __restrict__ spec, const double* __restrict__ ktab, int nx, int ny, int nz,
+                               double inv_ntot)
+{
+#pragma clang fp contract(off)
+  const int nxh = nx / 2 + 1;
+  const long long N = (long long)nxh * ny * nz;
+  const double* xk2 = ktab;
+  const double* tx = xk2 + nx;
+  const double* yk2 = tx + nx;
+  const double* ty = yk2 + ny;
+  const double* zk2 = ty + ny;
+  const double* tz = zk2 + nz;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < N;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % nxh);
+    const long long r = q / nxh;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    const double tyz = ty[j] * tz[k], txz = tx[i] * tz[k], txy = tx[i] * ty[j];
+    const double K = (xk2[i] * (tyz * tyz) + yk2[j] * (txz * txz)) + zk2[k] * (txy * txy);
+    float2 v = spec[q];
+    if (K < 1.0e-16) {
+      v.x = 0.0f;
+      v.y = 0.0f;
+    } else {
+      v.x = (float)(((double)v.x * inv_ntot) / (-K));
+      v.y = (float)(((double)v.y * inv_ntot) / (-K));
+    }
+    spec[q] = v;
+  }
+}
+
+int poisson_prepare_f32(x3d_ctx* c)
+{
+  if (!c->fft_ready) return X3D_ERR_FFT;
+  if (!c->fft32_ready) {
+    if (hipfftPlan3d(&c->fft_r2c32, c->nz, c->ny, c->nx, HIPFFT_R2C) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+    if (hipfftPlan3d(&c->fft_c2r32, c->nz, c->ny, c->nx, HIPFFT_C2R) != HIPFFT_SUCCESS) {
+      hipfftDestroy(c->fft_r2c32);
+      return X3D_ERR_FFT;
+    }
+    c->fft32_ready = true;
+  }
+  return X3D_OK;
+}
+
+int poisson_solve_f32(x3d_ctx* c, float* rhs)
+{
+  const int rc = poisson_prepare_f32(c);
+  if (rc) return rc;
+  if (hipfftSetStream(c->fft_r2c32, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  if (hipfftSetStream(c->fft_c2r32, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  // the FP64 spectrum buffer holds the FP32 one
+  float2* spec = reinterpret_cast<float2*>(c->spec);
+  if (hipfftExecR2C(c->fft_r2c32, rhs, (hipfftComplex*)spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  const long long N = (long long)(c->nx / 2 + 1) * c->ny * c->nz;
+  long long blocks = (N + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  const double inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
+  hipLaunchKernelGGL(k_spectral_f32, dim3((unsigned)blocks), dim3(256), 0, c->stream, spec, c->ktab, c->nx, c->ny,
+                     c->nz, inv_ntot);
+  X3D_CHECK_HIP(hipGetLastError());
+  if (hipfftExecC2R(c->fft_c2r32, (hipfftComplex*)spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  return X3D_OK;
 }
 
 int poisson_solve(x3d_ctx* c, double* rhs)
