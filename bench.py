@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--cpu-n", type=int, default=256, help="grid of the CPU-baseline sample step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="fp32,batch,large",
+                    help="comma list of extra forms to measure beside plain FP64: fp32 (precision "
+                         "form at n^3), batch (96 uneven grids, FP64 and FP32), large (FP32 at "
+                         "1024x768x768 beside FP64 at 768^3); empty for none")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch a plain device copy of one field (counter calibration) "
                          "and whole steps, for rocprofv3 --pmc runs")
@@ -103,6 +107,11 @@ def cpu_baseline(n: int):
     }
 
 
+def log(msg: str) -> None:
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def rel_errs(a, b):
     """max and L2 norms of a - b relative to b, in FP64."""
     a = a.double()
@@ -141,14 +150,15 @@ def time_form(ctx, f, step_fn, timed_fn, args):
     return ms_step, gpu_ms_step, passes
 
 
-def roofline_of(passes, n, gpu_ms_step, suffix):
+def roofline_of(passes, n, gpu_ms_step, suffix, npts=None):
     """The slowest line pass (the Poisson pass is several hipFFT kernels and is
     reported beside it) against the HBM peak, plus the whole step's fraction."""
+    npts = npts or n ** 3
     dom = max((k for k in passes if k != "poisson"), key=lambda k: passes[k]["ms"])
     kms = passes[dom]["ms"]
-    alg_bytes = int(passes[dom]["bytes_per_point"] * n ** 3)
+    alg_bytes = int(passes[dom]["bytes_per_point"] * npts)
     achieved = alg_bytes / (kms / 1e3) / 1e9
-    step_bytes = sum(p["bytes_per_point"] for p in passes.values()) * n ** 3
+    step_bytes = sum(p["bytes_per_point"] for p in passes.values()) * npts
     kern = KERNEL_OF_PASS[dom].replace("double", "float") if suffix else KERNEL_OF_PASS[dom]
     return {
         "bound": "hbm",
@@ -164,6 +174,118 @@ def roofline_of(passes, n, gpu_ms_step, suffix):
         "step_frac": round(step_bytes / (gpu_ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "passes": passes,
     }
+
+
+def measure_fp32(ctx, u0, args, ms_step):
+    """The precision form on the same grid, from the same fields rounded to FP32."""
+    import torch
+    n = ctx.nx
+    f32 = [t.float() for t in u0] + [ctx.field32() for _ in range(7)]
+    ms32, gpu_ms32, passes32 = time_form(ctx, f32, ctx.step_f32, ctx.step_timed_f32, args)
+    # accuracy of one step and of the 18 operators against the plain form
+    # (itself within 1e-12 of the CPU oracle)
+    g64 = [t.clone() for t in u0] + [ctx.field() for _ in range(7)]
+    g32 = [t.float() for t in u0] + [ctx.field32() for _ in range(7)]
+    ctx.step(*g64, gdt=0.0)
+    ctx.step_f32(*g32, gdt=0.0)
+    torch.cuda.synchronize()
+    step_err = {name: rel_errs(g32[i], g64[i]) for i, name in ((0, "ux"), (3, "dux"), (6, "pp3"))}
+    del g64, g32
+    # per operator: the FP32 arithmetic alone (FP64 applied to the same
+    # FP32-rounded field), and with the rounding of the field to FP32
+    op_err, op_err_in = {}, {}
+    o64, r64, o32, i32 = ctx.field(), ctx.field(), ctx.field32(), u0[0].float()
+    i64 = i32.double()
+    from x3div_amd import OPS
+    for name in OPS:
+        ctx.apply(name, o64, i64)
+        ctx.apply(name, r64, u0[0])
+        ctx.apply_f32(name, o32, i32)
+        torch.cuda.synchronize()
+        op_err[name] = rel_errs(o32, o64)["max"]
+        op_err_in[name] = rel_errs(o32, r64)["max"]
+    del o64, r64, o32, i32, i64, f32
+    return {"value": round(n ** 3 / (ms32 / 1e3) / 1e6, 2), "ms_per_step": round(ms32, 4),
+            "dtype": "f32 line solves and FFTs, f64 spectral division",
+            "speedup_vs_fp64": round(ms_step / ms32, 3),
+            "roofline": roofline_of(passes32, n, gpu_ms32, "_f32"),
+            "step_rel_err_vs_fp64": step_err,
+            "operator_max_rel_err_vs_fp64_same_input": op_err,
+            "operator_max_rel_err_vs_fp64_incl_input_rounding": op_err_in}
+
+
+BATCH_SIZES = (32, 48, 64, 96, 128, 160)
+
+
+def batch_dims(seed: int = 2024, ngrids: int = 96):
+    """96 grids, each dimension drawn uniformly from BATCH_SIZES (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [tuple(int(v) for v in rng.choice(BATCH_SIZES, 3)) for _ in range(ngrids)]
+
+
+def measure_batch(args):
+    """The batching form: 96 uneven grids with 8-mode Fourier fields, tilings
+    from the autotuner, the batched step captured in a HIP graph and replayed."""
+    import torch
+    from x3div_amd import X3DBatch
+    dims = batch_dims()
+    out = {}
+    for prec, tag in ((8, "batch96_fp64"), (4, "batch96_fp32")):
+        b = X3DBatch(dims, precision=prec)
+        b.init_fields(kind=2, seed=77)
+        report = b.autotune(reps=2)
+        b.graph_capture(gdt=0.0)
+        for _ in range(args.warmup):
+            b.graph_launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.graph_launch()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        pts = b.total_points
+        out[tag] = {"value": round(pts / (ms / 1e3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms, 4),
+                    "grids": len(dims), "points": pts,
+                    "dims_seed": 2024, "fields": "8 seeded Fourier modes per component, amplitude 1/8 each",
+                    "autotune": report.strip().splitlines()[-1] if report.strip() else ""}
+        b.close()
+        torch.cuda.empty_cache()
+    return out
+
+
+def measure_large(args):
+    """FP32 precision form at 1024x768x768 beside plain FP64 at 768^3."""
+    import numpy as np
+    import torch
+    from x3div_amd import X3D
+    out = {}
+    L = 2 * np.pi
+    for (nx, ny, nz), prec, tag in (((1024, 768, 768), 4, "large_fp32_1024x768x768"),
+                                    ((768, 768, 768), 8, "fp64_768cubed")):
+        ctx = X3D(nx, ny, nz, L, L, L)
+        u = [ctx.field() for _ in range(3)]
+        ctx.init_fields(*u, kind=1, seed=1234, amp=1e-3)
+        if prec == 4:
+            f = [t.float() for t in u] + [ctx.field32() for _ in range(7)]
+            del u
+            torch.cuda.empty_cache()
+            ms, gpu_ms, passes = time_form(ctx, f, ctx.step_f32, ctx.step_timed_f32, args)
+            suffix = "_f32"
+        else:
+            f = u + [ctx.field() for _ in range(7)]
+            ms, gpu_ms, passes = time_form(ctx, f, ctx.step, ctx.step_timed, args)
+            suffix = ""
+        pts = nx * ny * nz
+        rl = roofline_of(passes, 0, gpu_ms, suffix, npts=pts)
+        out[tag] = {"value": round(pts / (ms / 1e3) / 1e6, 2), "unit": "Mpts/s", "ms_per_step": round(ms, 4),
+                    "grid": [nx, ny, nz], "dtype": "f32 line solves and FFTs, f64 spectral division"
+                    if prec == 4 else "f64", "roofline_frac": rl["frac"], "step_frac": rl["step_frac"],
+                    "dominant": rl["kernel"]}
+        ctx.close()
+        del f
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -193,45 +315,26 @@ def main():
     value = n ** 3 / (ms_step / 1e3) / 1e6
     roofline = roofline_of(passes, n, gpu_ms_step, "")
 
-    # precision form: FP32 fields and line solves, FP64 spectral division,
-    # started from the same fields rounded to FP32
-    f32 = [t.float() for t in u0] + [ctx.field32() for _ in range(7)]
-    ms32, gpu_ms32, passes32 = time_form(ctx, f32, ctx.step_f32, ctx.step_timed_f32, args)
-    # accuracy of one step and of the 18 operators against the plain form
-    # (itself within 1e-12 of the CPU oracle)
-    g64 = [t.clone() for t in u0] + [ctx.field() for _ in range(7)]
-    g32 = [t.float() for t in u0] + [ctx.field32() for _ in range(7)]
-    ctx.step(*g64, gdt=0.0)
-    ctx.step_f32(*g32, gdt=0.0)
-    torch.cuda.synchronize()
-    step_err = {name: rel_errs(g32[i], g64[i]) for i, name in ((0, "ux"), (3, "dux"), (6, "pp3"))}
-    del g64, g32
-    op_err = {}
-    o64, o32, i32 = ctx.field(), ctx.field32(), u0[0].float()
-    from x3div_amd import OPS
-    for name in OPS:
-        ctx.apply(name, o64, u0[0])
-        ctx.apply_f32(name, o32, i32)
-        torch.cuda.synchronize()
-        op_err[name] = rel_errs(o32, o64)["max"]
-    del o64, o32, i32
-    forms = {
-        "fp64": {"value": round(value, 2), "ms_per_step": round(ms_step, 4), "dtype": "f64",
-                 "step_frac": roofline["step_frac"]},
-        "fp32": {"value": round(n ** 3 / (ms32 / 1e3) / 1e6, 2), "ms_per_step": round(ms32, 4),
-                 "dtype": "f32 line solves and FFTs, f64 spectral division",
-                 "speedup_vs_fp64": round(ms_step / ms32, 3),
-                 "roofline": roofline_of(passes32, n, gpu_ms32, "_f32"),
-                 "step_rel_err_vs_fp64": step_err,
-                 "operator_max_rel_err_vs_fp64": op_err},
-    }
+    extra = [x for x in args.forms.split(",") if x]
+    forms = {"fp64": {"value": round(value, 2), "ms_per_step": round(ms_step, 4), "dtype": "f64",
+                      "step_frac": roofline["step_frac"]}}
+    log(f"fp64 {n}^3: {ms_step:.3f} ms/step")
+    if "fp32" in extra:
+        forms["fp32"] = measure_fp32(ctx, u0, args, ms_step)
+        log(f"fp32 {n}^3: {forms['fp32']['ms_per_step']:.3f} ms/step")
     ctx.close()
-    del f, f32, u0
+    del f, u0
     torch.cuda.empty_cache()
+    if "batch" in extra:
+        forms.update(measure_batch(args))
+        log("batch forms done")
+    if "large" in extra:
+        forms.update(measure_large(args))
+        log("large grids done")
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_n)
     out = {
-        "metric": "Mpts/s per Euler timestep (plain FP64)",
+        "metric": "Mpts/s per Euler timestep at 512\u00b3 (plain FP64 vs FP32/split-BF16) + HBM GB/s",
         "value": round(value, 2),
         "unit": "Mpts/s",
         "n_gpus": 1,

@@ -35,7 +35,8 @@ extern "C" {
 #define X3D_ERR_HIP (-3)         /* a HIP runtime call failed                */
 #define X3D_ERR_FFT (-4)         /* a hipFFT call failed                     */
 #define X3D_ERR_ALLOC (-5)       /* device allocation failed                 */
-#define X3D_ERR_UNSUPPORTED (-6) /* line length not factorable for the kernels */
+#define X3D_ERR_UNSUPPORTED (-6) /* line length not factorable for the kernels, or a
+                                    field of the requested precision of 4 GiB or more */
 
 /* Operator instances (src/x3d_operator_1d.f90:33-47), in this order. */
 enum {

@@ -144,3 +144,32 @@ def test_step_timed_fp32_matches_step(grid):
         outs.append(f)
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+def test_large_grid_fp32_only(torch_cuda):
+    """1024 x 768 x 768: FP32 fields fit the kernels' 32-bit offsets, FP64 ones
+    (4.8 GB) do not, so the FP64 entry points refuse and FP32 runs.  Checked
+    by the closed form: derx of sin(2 pi x / L) is (2 pi / L) lambda cos, with
+    lambda the scheme's modified-wavenumber ratio, here within FP32 round-off
+    of the exact derivative."""
+    import torch
+    from x3div_amd import X3D, X3DError
+    nx, ny, nz = 1024, 768, 768
+    print(f"\n[fp32 large {nx}x{ny}x{nz}]", flush=True)
+    L = 2 * np.pi
+    ctx = X3D(nx, ny, nz, L, L, L)
+    x = torch.arange(nx, dtype=torch.float64, device="cuda") * (L / nx)
+    u = torch.sin(x).float().expand(nz, ny, nx).contiguous()
+    t = ctx.field32()
+    ctx.derx_f32(t, u)
+    torch.cuda.synchronize()
+    ref = torch.cos(x).float()
+    err = float((t - ref).abs().max())
+    assert err < 2e-5, err
+    u64 = torch.zeros((nz, ny, nx), dtype=torch.float64, device="cuda")
+    t64 = torch.zeros_like(u64)
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.derx(t64, u64)
+    del u64, t64
+    ctx.close()
+    torch.cuda.empty_cache()

@@ -348,7 +348,7 @@ const char* x3d_strerror(int code)
     case X3D_ERR_HIP: return "HIP runtime error";
     case X3D_ERR_FFT: return "hipFFT error";
     case X3D_ERR_ALLOC: return "device allocation failed";
-    case X3D_ERR_UNSUPPORTED: return "line length not supported by the line kernels";
+    case X3D_ERR_UNSUPPORTED: return "not supported: line length not tiled by the line kernels, FP64 field of 4 GiB or more, or replaced operator tables";
     default: return "unknown error";
   }
 }
@@ -366,9 +366,12 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   *out = nullptr;
   if (nx < 8 || ny < 8 || nz < 8) return X3D_ERR_SIZE;
   if (!(xlx > 0) || !(yly > 0) || !(zlz > 0)) return X3D_ERR_ARG;
-  // the line kernels address a field with 32-bit byte offsets
-  if ((unsigned long long)nx * ny * nz * sizeof(double) > 0xFFFFFFFFull) return X3D_ERR_UNSUPPORTED;
+  // the line kernels address a field with 32-bit byte offsets: FP32 fields
+  // must stay below 4 GiB, and FP64 entry points need FP64 fields below it
+  const unsigned long long npts = (unsigned long long)nx * ny * nz;
+  if (npts * sizeof(float) > 0xFFFFFFFFull) return X3D_ERR_UNSUPPORTED;
   x3d_ctx* c = new x3d_ctx();
+  c->f64_lines = npts * sizeof(double) <= 0xFFFFFFFFull;
   c->nx = nx; c->ny = ny; c->nz = nz;
   c->xlx = xlx; c->yly = yly; c->zlz = zlz;
   c->dx = xlx / (double)nx; c->dy = yly / (double)ny; c->dz = zlz / (double)nz;  // nXm = nX

@@ -235,6 +235,7 @@ static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out, bool periodic_fix)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS || !in || !out) return X3D_ERR_ARG;
+  if (!c->f64_lines) return X3D_ERR_UNSUPPORTED;
   const LinePlan& P = c->plan[dir];
   const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
   if (c->op[op].n != n || P.n != n) return X3D_ERR_SIZE;

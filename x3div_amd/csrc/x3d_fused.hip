@@ -459,6 +459,7 @@ static T* wk(x3d_ctx* c, int i) { return reinterpret_cast<T*>(c->work[i]); }
 template <typename T>
 int fused_momentum(x3d_ctx* c, T gdt, int euler, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz)
 {
+  if (sizeof(T) == 8 && !c->f64_lines) return X3D_ERR_UNSUPPORTED;
   X3D_TRY(ensure_work(c));
   T *tx = wk<T>(c, 5), *ty = wk<T>(c, 6), *tz = wk<T>(c, 7);
   X3D_TRY((transeq_pass<0, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0)));
@@ -473,6 +474,7 @@ int fused_momentum(x3d_ctx* c, T gdt, int euler, T* ux, T* uy, T* uz, T* dux, T*
 template <typename T>
 int fused_divergence(x3d_ctx* c, T* pp3, const T* ux, const T* uy, const T* uz)
 {
+  if (sizeof(T) == 8 && !c->f64_lines) return X3D_ERR_UNSUPPORTED;
   X3D_TRY(ensure_work(c));
   T *pp1 = wk<T>(c, 0), *pgy1 = wk<T>(c, 1), *pgz1 = wk<T>(c, 2), *dd2 = wk<T>(c, 3), *upi2 = wk<T>(c, 4);
   X3D_TRY((div_pass<0, T>(c, ux, uy, uz, pp1, pgy1, pgz1)));
@@ -487,6 +489,7 @@ int fused_divergence(x3d_ctx* c, T* pp3, const T* ux, const T* uy, const T* uz)
 template <typename T>
 int fused_gradp(x3d_ctx* c, T* px, T* py, T* pz, const T* pp3, T* ux, T* uy, T* uz, int corr)
 {
+  if (sizeof(T) == 8 && !c->f64_lines) return X3D_ERR_UNSUPPORTED;
   X3D_TRY(ensure_work(c));
   T *ppi3 = wk<T>(c, 0), *pgz3 = wk<T>(c, 1), *ppi2 = wk<T>(c, 2), *pgy2 = wk<T>(c, 3), *pgzi2 = wk<T>(c, 4);
   X3D_TRY((grad_pass<2, T>(c, pp3, nullptr, nullptr, ppi3, pgz3, nullptr, nullptr, nullptr, nullptr, 0)));

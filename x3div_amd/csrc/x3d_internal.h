@@ -65,6 +65,7 @@ struct x3d_ctx {
   int nx = 0, ny = 0, nz = 0;
   double xlx = 1.0, yly = 1.0, zlz = 1.0, dx = 0.0, dy = 0.0, dz = 0.0;
   int device = 0;
+  bool f64_lines = true;  // FP64 fields below 4 GiB (32-bit offsets of the line kernels)
   hipStream_t stream = nullptr;
   x3d::Scheme sc[3];
   x3d::OpHost op[X3D_NOPS];
