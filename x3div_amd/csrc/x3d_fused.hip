@@ -31,7 +31,7 @@ struct TranseqArgs {
 };
 
 template <typename T, int M, int DIR>
-__device__ __forceinline__ void transeq_body(const TranseqArgs<T>& A, int blk)
+__device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ A, int blk)
 {
   T* const ux = A.ux; T* const uy = A.uy; T* const uz = A.uz;
   T* const dux = A.dux; T* const duy = A.duy; T* const duz = A.duz;
@@ -131,7 +131,7 @@ struct PassArgs {
 };
 
 template <typename T, int M, int DIR>
-__device__ __forceinline__ void div_body(const PassArgs<T>& A, int blk)
+__device__ __forceinline__ void div_body(const PassArgs<T>& __restrict__ A, int blk)
 {
   const T* const a = A.a; const T* const b = A.b; const T* const c = A.c;
   T* const o1 = A.o1; T* const o2 = A.o2; T* const o3 = A.o3;
@@ -217,7 +217,7 @@ __device__ __forceinline__ void div_body(const PassArgs<T>& A, int blk)
 // X: o1 = Dpv(a), o2 = Ipv(b), o3 = Ipv(c)          a, b, c = ppi2, pgy2, pgzi2
 //    if corr: u_k = u_k - o_k (src/navier.f90:84-86)
 template <typename T, int M, int DIR>
-__device__ __forceinline__ void grad_body(const PassArgs<T>& A, int blk)
+__device__ __forceinline__ void grad_body(const PassArgs<T>& __restrict__ A, int blk)
 {
   const T* const a = A.a; const T* const b = A.b; const T* const c = A.c;
   T* const o1 = A.o1; T* const o2 = A.o2; T* const o3 = A.o3;
