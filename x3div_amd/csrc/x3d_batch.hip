@@ -66,7 +66,17 @@ int batch_step_body(x3d_batch* b, double gdt)
 
 // Line lengths that this y/z tiling candidate fits, for the autotuner: the
 // chunk length M, chunk groups per wavefront G, line groups per work-group Wl.
-std::vector<LinePlan> candidates(int n, int prec)
+// LDS bytes of the largest pass (transeq, with its keep slots) for a y/z
+// tiling: solver aggregates (4 Wc + 2 lines per work-group when Wc > 1) plus
+// (2 M + 4) slots per thread.
+size_t plan_lds_bytes(const LinePlan& p, int prec)
+{
+  const size_t lpw = (size_t)p.Wl * p.LW, threads = 64u * p.Wc * p.Wl;
+  const size_t solver = p.Wc > 1 ? 4 * p.Wc * lpw + 2 * lpw : 0;
+  return (size_t)prec * (solver + threads * (size_t)(2 * p.M + 4));
+}
+
+std::vector<LinePlan> candidates(int n, int prec, size_t lds_max)
 {
   static const int ms64[] = {1, 2, 3, 4, 5, 6, 8, 12, 16};
   static const int ms32[] = {1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32};
@@ -85,7 +95,7 @@ std::vector<LinePlan> candidates(int n, int prec)
         if (wc * wl > 8) continue;
         LinePlan p;
         p.n = n; p.M = m; p.C = c; p.G = g; p.Wc = wc; p.Wl = wl; p.LW = 64 / g; p.line_fast = true;
-        out.push_back(p);
+        if (plan_lds_bytes(p, prec) <= lds_max) out.push_back(p);
       }
     }
   }
@@ -277,6 +287,10 @@ int x3d_batch_autotune(x3d_batch* b, int reps, char* report, size_t report_len)
     *ms = t / reps;
     return rc;
   };
+  int dev = 0, lds = 65536;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  const size_t lds_max = (size_t)lds;
   double best_ms = 0.0;
   int rc = time_step(&best_ms);
   if (rc) return rc;
@@ -292,7 +306,7 @@ int x3d_batch_autotune(x3d_batch* b, int reps, char* report, size_t report_len)
       auto plan_of = [&](int i) -> LinePlan& { return b->prec == 8 ? b->g[i]->fplan[dir] : b->g[i]->fplan32[dir]; };
       const LinePlan keep = plan_of(kv.second[0]);
       LinePlan best = keep;
-      for (const LinePlan& cand : candidates(kv.first, b->prec)) {
+      for (const LinePlan& cand : candidates(kv.first, b->prec, lds_max)) {
         if (cand.M == keep.M && cand.G == keep.G && cand.Wc == keep.Wc && cand.Wl == keep.Wl) continue;
         for (int i : kv.second) plan_of(i) = cand;
         b->built = false;

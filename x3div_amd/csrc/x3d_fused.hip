@@ -49,6 +49,13 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
   T* const Ud = DIR == 0 ? ux : (DIR == 1 ? uy : uz);
   XBlock<T, M> xb;
   if constexpr (DIR == 0) xb = x_block<T, M>(geo, t, lds);
+  // y/z: the u_d window (the same for the three components) and, with the
+  // Euler update, each u_c chunk are kept in per-thread LDS slots from their
+  // loads to their later uses instead of being loaded again: strided loads
+  // cost address translation per row, LDS slots do not.  Slot q of thread i
+  // is keep[q * blockDim.x + i], after the solver's words.
+  T* const keep = lds + line_lds_words(t.Wc, t.LPW);
+  const int nth = blockDim.x, tid = threadIdx.x;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     // components one after another: no loads hoisted across them
@@ -66,8 +73,24 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
         x_load_window<T, M, H, H>(Uc, xb, t, wc);
         x_load_window<T, M, H, H>(Ud, xb, t, wd);
       } else {
-        load_window<T, M, H, H, DIR != 0>(Uc, t, n, wc);
-        load_window<T, M, H, H, DIR != 0>(Ud, t, n, wd);
+        if (c == 0) {
+          load_window<T, M, H, H, true>(Ud, t, n, wd);
+#pragma unroll
+          for (int q = 0; q < NW; ++q) keep[q * nth + tid] = wd[q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < NW; ++q) wd[q] = keep[q * nth + tid];
+        }
+        if (c == DIR) {
+#pragma unroll
+          for (int q = 0; q < NW; ++q) wc[q] = wd[q];
+        } else {
+          load_window<T, M, H, H, true>(Uc, t, n, wc);
+        }
+        if constexpr (DIR == 2) {
+#pragma unroll
+          for (int k = 0; k < M; ++k) keep[(NW + k) * nth + tid] = wc[H + k];
+        }
       }
       rhs_chunk<K_D, T, M, H>(wc, cf, v2);
 #pragma unroll
@@ -91,7 +114,7 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
       for (int k = 0; k < M; ++k) v1[k] = v1[k] + ud[k] * v2[k];
     } else {
 #pragma unroll
-      for (int k = 0; k < M; ++k) v1[k] = v1[k] + ld(Ud, t.at(t.j0 + k)) * v2[k];
+      for (int k = 0; k < M; ++k) v1[k] = v1[k] + keep[(H + k) * nth + tid] * v2[k];
     }
     if constexpr (DIR == 0) {
       x_store<T, M>(DUc, xb, v1);
@@ -104,7 +127,7 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
           const uint32_t at = t.at(t.j0 + k);
           const T d = ld(DUc, at) + (ld(TMc, at) + v1[k]);
           st(DUc, at, d);
-          if (euler) st(Uc, at, gdt * d + ld(Uc, at));
+          if (euler) st(Uc, at, gdt * d + keep[(NW + k) * nth + tid]);
         }
       }
     }
@@ -333,12 +356,20 @@ __global__ __launch_bounds__(512) void k_grad_batch(const PassArgs<T>* __restric
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Work-groups above 64 KiB of dynamic LDS (up to the CU's 160 KiB) need the
+// kernel's limit raised once.
+static void allow_lds(const void* fn, size_t bytes)
+{
+  if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 template <typename T>
 struct DirSet {
   Geom geo;
   Tiling tl;
   dim3 grid, block;
   size_t smem;
+  size_t smem_tq;  // transeq: plus the y/z keep slots
   int M;
   Coef<T> cf_d, cf_dvp, cf_dpv, cf_ivp, cf_ipv;  // scaled by 1/gamma
   Circ<T> cc_d, cc_s, cc_i;
@@ -358,6 +389,10 @@ static DirSet<T> dirset(const x3d_ctx* c, int dir)
   const LinePlan& P = fused_plan<T>(c, dir);
   launch_dims_plan(c, dir, P, &s.geo, &s.tl, &s.grid, &s.block, &s.smem, sizeof(T));
   s.M = P.M;
+  // keep slots of transeq: the u_d window (M + 4) and, along z, the u_c chunk
+  s.smem_tq = s.smem;
+  if (P.line_fast)
+    s.smem_tq += sizeof(T) * (size_t)s.block.x * (size_t)(P.M + 4 + (dir == 2 ? P.M : 0));
   const Scheme& sc = c->sc[dir];
   // coefficients in FP64, rounded once to T
   auto scaled = [](Coef<double> k, double alfa) {
@@ -416,7 +451,10 @@ static int transeq_pass(x3d_ctx* c, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz,
 {
   const DirSet<T> s = dirset<T>(c, DIR);
   const TranseqArgs<T> A{ux, uy, uz, dux, duy, duz, tx, ty, tz, s.cf_d, s.cc_d, s.geo, s.tl, gdt, euler};
-  X3D_MSWITCH_T(T, s.M, hipLaunchKernelGGL((k_transeq<T, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
+  X3D_MSWITCH_T(T, s.M, {
+    allow_lds((const void*)k_transeq<T, MM, DIR>, s.smem_tq);
+    hipLaunchKernelGGL((k_transeq<T, MM, DIR>), s.grid, s.block, s.smem_tq, c->stream, A);
+  });
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
@@ -569,7 +607,7 @@ int batch_build(x3d_batch* b, double gdt)
         }
         block0.push_back(block0.back() + (int)s.grid.x);
         L.block = s.block;
-        L.smem = s.smem;
+        L.smem = pass < 3 ? s.smem_tq : s.smem;
       }
       L.ngroup = (int)kv.second.size();
       L.args = pass < 3 ? put(ta.data(), ta.size() * sizeof(TranseqArgs<T>))
@@ -606,8 +644,10 @@ static int batch_launch(x3d_batch* b, const x3d_batch::Launch& L)
   switch (L.pass) {
 #define X3D_BT(P, K, DIR, LIST)                                                                             \
   case P:                                                                                                 \
-    X3D_MSWITCH_T(T, L.M, hipLaunchKernelGGL((K<T, MM, DIR>), L.grid, L.block, L.smem, st, LIST, block0, \
-                                             L.ngroup));                                                  \
+    X3D_MSWITCH_T(T, L.M, {                                                                               \
+      allow_lds((const void*)K<T, MM, DIR>, L.smem);                                                      \
+      hipLaunchKernelGGL((K<T, MM, DIR>), L.grid, L.block, L.smem, st, LIST, block0, L.ngroup);           \
+    });                                                                                                   \
     break;
     X3D_BT(0, k_transeq_batch, 0, ta) X3D_BT(1, k_transeq_batch, 1, ta) X3D_BT(2, k_transeq_batch, 2, ta)
     X3D_BT(3, k_div_batch, 0, pa) X3D_BT(4, k_div_batch, 1, pa) X3D_BT(5, k_div_batch, 2, pa)
