@@ -778,7 +778,8 @@ int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double*
 
 namespace {
 // Pass names and algorithmic HBM bytes per grid point (FP64: each field read or
-// written once per pass).  transeq Z reads u, du, tmp and writes du, u; gradp X
+// written once per pass).  transeq Y reads u, du and writes tmp = du + its
+// part; transeq Z reads u, tmp and writes du, u; gradp X
 // reads three gradients and u and writes p gradients and u.  The Poisson figure
 // is one r2c read/write, one in-place spectral update and one c2r read/write.
 struct PassInfo {
@@ -786,7 +787,7 @@ struct PassInfo {
   int fields;
 };
 constexpr PassInfo kPass[X3D_NPASS] = {
-    {"transeq_x", 6}, {"transeq_y", 6}, {"transeq_z", 15}, {"divergence_x", 6}, {"divergence_y", 5},
+    {"transeq_x", 6}, {"transeq_y", 9}, {"transeq_z", 12}, {"divergence_x", 6}, {"divergence_y", 5},
     {"divergence_z", 3}, {"poisson", 6}, {"gradp_z", 3}, {"gradp_y", 5}, {"gradp_x", 12}};
 }  // namespace
 

@@ -16,8 +16,8 @@
 namespace x3d {
 
 // ---- transeq: r_c = D(u_d u_c) + u_d D(u_c), d = DIR ----------------------
-// X: du_c = r_c; Y: tmp_c = r_c;
-// Z: du_c = du_c + (tmp_c + r_c); if euler: u_c = gdt du_c + u_c.
+// X: du_c = r_c; Y: tmp_c = du_c + r_c;
+// Z: du_c = tmp_c + r_c; if euler: u_c = gdt du_c + u_c.
 // (src/transeq.f90:100-115, 120-134, 139-153, 158-177 per component)
 template <typename T>
 struct TranseqArgs {
@@ -119,13 +119,26 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
     if constexpr (DIR == 0) {
       x_store<T, M>(DUc, xb, v1);
     } else if constexpr (DIR == 1) {
-      store_chunk<T, M>(TMc, t, v1);
+      // tmp_c = du_c (x part) + r_c: the sum is formed here, where a strided
+      // row stays inside one page, rather than in the z pass
+      // (all loads before the first store: the stores may alias as far as
+      // the compiler knows, and would otherwise serialise the loads)
+      if (t.active) {
+        T a[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) a[k] = ld(DUc, t.at(t.j0 + k));
+#pragma unroll
+        for (int k = 0; k < M; ++k) st(TMc, t.at(t.j0 + k), a[k] + v1[k]);
+      }
     } else {
       if (t.active) {
+        T a[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) a[k] = ld(TMc, t.at(t.j0 + k));
 #pragma unroll
         for (int k = 0; k < M; ++k) {
           const uint32_t at = t.at(t.j0 + k);
-          const T d = ld(DUc, at) + (ld(TMc, at) + v1[k]);
+          const T d = a[k] + v1[k];
           st(DUc, at, d);
           if (euler) st(Uc, at, gdt * d + keep[(NW + k) * nth + tid]);
         }

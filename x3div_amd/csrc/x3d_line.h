@@ -184,11 +184,35 @@ __device__ __forceinline__ void load_window(const T* __restrict__ in, const Line
       win[HL + M - 1 + h] = __shfl(win[HL + idx], src);
     }
   } else {
+    // strided lines: the chunk points from memory; each halo point from the
+    // lane group holding that chunk in this wavefront, and from memory only
+    // in the groups at the wavefront's ends (their neighbours live in other
+    // wavefronts), so the masked loads touch a quarter of the rows
 #pragma unroll
-    for (int q = 0; q < M + HL + HR; ++q) {
-      int j = t.j0 - HL + q;
-      j = j < 0 ? j + n : (j >= n ? j - n : j);
-      win[q] = ld(in, t.at(j));
+    for (int k = 0; k < M; ++k) win[HL + k] = ld(in, t.at(t.j0 + k));
+#pragma unroll
+    for (int h = 1; h <= HL; ++h) {
+      const int d = (h + M - 1) / M;  // chunks to the left
+      const int idx = M * d - h;      // index inside that chunk
+      T v = __shfl(win[HL + idx], t.lane_of(t.g >= d ? t.g - d : t.g));
+      if (t.g < d) {
+        int j = t.j0 - h;
+        j = j < 0 ? j + n : j;
+        v = ld(in, t.at(j));
+      }
+      win[HL - h] = v;
+    }
+#pragma unroll
+    for (int h = 1; h <= HR; ++h) {
+      const int d = (h + M - 1) / M;  // chunks to the right
+      const int idx = h - 1 - M * (d - 1);
+      T v = __shfl(win[HL + idx], t.lane_of(t.g + d < t.G ? t.g + d : t.g));
+      if (t.g + d >= t.G) {
+        int j = t.j0 + M - 1 + h;
+        j = j >= n ? j - n : j;
+        v = ld(in, t.at(j));
+      }
+      win[HL + M - 1 + h] = v;
     }
   }
 }
