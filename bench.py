@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_FP16_PEAK_TFLOPS = 2500.0  # MI355X dense fp16 matrix peak
 
 
 def parse():
@@ -204,8 +205,36 @@ def measure_fp32(ctx, u0, args, ms_step):
         torch.cuda.synchronize()
         op_err[name] = rel_errs(o32, o64)["max"]
         op_err_in[name] = rel_errs(o32, r64)["max"]
+    # the dense-circulant matrix-core variant of the three derivatives against
+    # the FP32 line solves: time per application, MFMA rate, error vs FP64
+    mf = {}
+    if n % 128 == 0 and n <= 1024:
+        for name, fn in (("derx", ctx.derx_f32), ("dery", ctx.dery_f32), ("derz", ctx.derz_f32)):
+            ctx.apply_circ_mfma(name, o32, i32)
+            fn(o32, i32)
+            torch.cuda.synchronize()
+            t = {}
+            for tag, call in (("mfma", lambda: ctx.apply_circ_mfma(name, o32, i32)), ("thomas", lambda: fn(o32, i32))):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(ctx.stream)
+                for _ in range(args.kernel_reps):
+                    call()
+                e1.record(ctx.stream)
+                torch.cuda.synchronize()
+                t[tag] = e0.elapsed_time(e1) / args.kernel_reps
+            ctx.apply_circ_mfma(name, o32, i32)
+            ctx.apply(name, o64, i64)
+            torch.cuda.synchronize()
+            flops = 3 * 2 * n * n ** 3
+            mf[name] = {"mfma_ms": round(t["mfma"], 4), "thomas_f32_ms": round(t["thomas"], 4),
+                        "mfma_TFLOPs": round(flops / (t["mfma"] / 1e3) / 1e12, 1),
+                        "mfma_frac_of_fp16_peak": round(flops / (t["mfma"] / 1e3) / 1e12 / MFMA_FP16_PEAK_TFLOPS, 4),
+                        "mfma_hbm_GBps": round(8 * n ** 3 / (t["mfma"] / 1e3) / 1e9, 1),
+                        "max_rel_err_vs_fp64": rel_errs(o32, o64)["max"]}
     del o64, r64, o32, i32, i64, f32
     return {"value": round(n ** 3 / (ms32 / 1e3) / 1e6, 2), "ms_per_step": round(ms32, 4),
+            "circulant_mfma": mf,
             "dtype": "f32 line solves and FFTs, f64 spectral division",
             "speedup_vs_fp64": round(ms_step / ms32, 3),
             "roofline": roofline_of(passes32, n, gpu_ms32, "_f32"),

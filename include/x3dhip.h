@@ -200,6 +200,15 @@ int x3d_derz_f32(x3d_ctx *ctx, float *tz, const float *uz, int op, int nx, int n
  * of src/x3d_derive.f90 and src/x3d_staggered.f90 with periodic BCs. */
 int x3d_apply_f32(x3d_ctx *ctx, int op, float *out, const float *in);
 int x3d_poisson_f32(x3d_ctx *ctx, float *rhs);
+/* Matrix-core variant of x3d_apply_f32: with periodic boundaries each
+ * operator is the dense circulant C = A^-1 B (built once per operator in
+ * FP64 from the reference's own periodic solve), applied to every line as a
+ * GEMM on v_mfma_f32_32x32x8_f16 in three split-precision terms
+ * (C_hi u_hi + C_hi u_lo + C_lo u_hi, fp16 halves after exact power-of-two
+ * scaling, FP32 accumulation); a few 1e-6 relative to the FP64 operator.  Needs the line length a
+ * multiple of 128 (at most 1024) and the line count a multiple of 32;
+ * out != in. */
+int x3d_apply_circ_mfma(x3d_ctx *ctx, int op, float *out, const float *in);
 int x3d_step_f32(x3d_ctx *ctx, double gdt, float *ux, float *uy, float *uz,
                  float *dux, float *duy, float *duz, float *pp3,
                  float *px, float *py, float *pz);

@@ -74,6 +74,7 @@ SIGNATURES = {
     "x3d_derz_f32": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
     "x3d_apply_f32": (_i, [_vp, _i, _dp, _dp]),
     "x3d_poisson_f32": (_i, [_vp, _dp]),
+    "x3d_apply_circ_mfma": (_i, [_vp, _i, _dp, _dp]),
     "x3d_step_f32": (_i, [_vp, _d] + [_dp] * 10),
     "x3d_step_timed_f32": (_i, [_vp, _d] + [_dp] * 10 + [C.POINTER(C.c_float)]),
     "x3d_graph_capture_step": (_i, [_vp, _d] + [_dp] * 10),

@@ -259,6 +259,11 @@ class X3D:
         """Any of the 18 operators, by instance name, along its own direction, in FP32."""
         check("x3d_apply_f32", self.lib.x3d_apply_f32(self._h, _op_id(name), self._f32(out), self._f32(inp)))
 
+    def apply_circ_mfma(self, name: str, out, inp):
+        """The operator as a dense circulant on the matrix cores (split bf16, FP32 sums)."""
+        check("x3d_apply_circ_mfma", self.lib.x3d_apply_circ_mfma(self._h, _op_id(name), self._f32(out),
+                                                                  self._f32(inp)))
+
     def poisson_f32(self, rhs):
         check("x3d_poisson_f32", self.lib.x3d_poisson_f32(self._h, self._f32(rhs)))
 

@@ -417,6 +417,8 @@ int x3d_destroy(x3d_ctx* c)
   for (hipEvent_t e : c->pass_ev)
     if (e) (void)hipEventDestroy(e);
   graph_release(c);
+  for (void* t : c->circ_tab)
+    if (t) (void)hipFree(t);
   delete c;
   return X3D_OK;
 }

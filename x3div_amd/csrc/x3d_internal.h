@@ -93,6 +93,9 @@ struct x3d_ctx {
   // per-pass timing of x3d_step_timed: event i is recorded after pass i
   hipEvent_t pass_ev[X3D_NPASS + 1] = {};
   bool timing = false;
+  // dense-circulant fragments of the matrix-core variant, per operator (made on first use)
+  void* circ_tab[X3D_NOPS] = {};
+  float circ_scale[X3D_NOPS] = {};  // 2^-s of the power-of-two scaling of each table
   // captured step (x3d_graph_capture_step*), replayed by x3d_graph_launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
