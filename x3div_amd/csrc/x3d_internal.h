@@ -84,6 +84,8 @@ struct x3d_ctx {
   bool fft32_ready = false;
   double2* spec = nullptr;          // (nx/2+1) * ny * nz complex
   double* ktab = nullptr;           // device 1-D tables, see poisson source
+  bool zsplit = false;              // 2-D hipFFT plans + fused z stage (nz a power of two)
+  void* ztw = nullptr;              // e^{-2 pi i t / nz}, t < nz (double2)
   std::vector<double> ax, bx, ay, by, az, bz;
   std::vector<double> xk2, yk2, zk2, exs, eys, ezs;
   // step scratch fields (allocated on first use)

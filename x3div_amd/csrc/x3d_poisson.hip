@@ -55,6 +55,224 @@ __global__ void k_spectral(double2* __restrict__ spec, const double* __restrict_
   }
 }
 
+// ---- fused z stage ---------------------------------------------------------
+// With nz a power of two the 3-D transform is split: hipFFT runs the 2-D
+// r2c/c2r over (x, y) of every z-plane, and one kernel does, per z-line of the
+// half spectrum, the forward length-nz DFT, the division above and the
+// inverse DFT, so the spectrum crosses HBM once for the three steps.  A
+// work-group takes 8 neighbouring x-columns of one y (128-byte rows for
+// FP64), holds the 8 lines in LDS and runs Stockham radix-8 passes (a final
+// radix-4 or radix-2 pass when log2 nz is not a multiple of 3) with twiddles
+// from a table of e^{-2 pi i t / nz}.
+
+template <typename C>
+struct CplxT;
+template <>
+struct CplxT<double2> { using R = double; };
+template <>
+struct CplxT<float2> { using R = float; };
+
+template <typename C>
+__device__ __forceinline__ C cmul(C a, double2 w, bool conj)
+{
+  using R = typename CplxT<C>::R;
+  const R wx = (R)w.x, wy = conj ? (R)-w.y : (R)w.y;
+  C r;
+  r.x = a.x * wx - a.y * wy;
+  r.y = a.x * wy + a.y * wx;
+  return r;
+}
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// multiply by -i (forward) or +i (inverse)
+template <typename C>
+__device__ __forceinline__ C cmi(C a, bool inv) { C r; if (inv) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; } return r; }
+
+// In-place DFT of length R (2, 4, 8) with sign -1 (forward) or +1 (inverse).
+template <typename C, int R>
+__device__ __forceinline__ void small_dft(C (&v)[R], bool inv)
+{
+  using T = typename CplxT<C>::R;
+  if constexpr (R == 2) {
+    const C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const C t2 = cadd(v[1], v[3]), t3 = cmi(csub(v[1], v[3]), inv);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  } else {
+    // X[2q] = DFT4(v_r + v_{r+4})[q], X[2q+1] = DFT4((v_r - v_{r+4}) W8^r)[q]
+    const T h = (T)0.70710678118654752440;
+    C a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      a[r] = cadd(v[r], v[r + 4]);
+      b[r] = csub(v[r], v[r + 4]);
+    }
+    // W8 = (h, -h) forward, (h, h) inverse; W8^2 = -i / +i; W8^3 = (-h, -h) / (-h, h)
+    {
+      const C x = b[1];
+      b[1].x = h * (x.x + (inv ? -x.y : x.y));
+      b[1].y = h * (x.y + (inv ? x.x : -x.x));
+    }
+    b[2] = cmi(b[2], inv);
+    {
+      const C x = b[3];
+      b[3].x = h * (-x.x + (inv ? -x.y : x.y));
+      b[3].y = h * (-x.y + (inv ? x.x : -x.x));
+    }
+    small_dft<C, 4>(a, inv);
+    small_dft<C, 4>(b, inv);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      v[2 * q] = a[q];
+      v[2 * q + 1] = b[q];
+    }
+  }
+}
+
+// One Stockham pass of radix R over the TI lines held in buf[k][TI]
+// (Ns = product of the radices already applied).
+template <typename C, int N, int R, int TI>
+__device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict__ tw, int Ns, bool inv)
+{
+  constexpr int NB = TI * N / R;             // butterflies in the work-group
+  constexpr int BPT = (NB + 255) / 256;      // per thread
+  C v[BPT][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + 256 * q;
+    if (b < NB) {
+      const int col = b % TI, jj = b / TI;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[(jj + r * (N / R)) * TI + col];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + 256 * q;
+    if (b < NB) {
+      const int col = b % TI, jj = b / TI;
+      const int kin = jj % Ns;
+      const int step = N / (Ns * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(kin * r * step) & (N - 1)], inv);
+      small_dft<C, R>(v[q], inv);
+      const int d = (jj / Ns) * Ns * R + kin;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(d + r * Ns) * TI + col] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename C, int N, int TI>
+__device__ __forceinline__ void line_dft(C* buf, const double2* __restrict__ tw, bool inv)
+{
+  int Ns = 1;
+  constexpr int L = __builtin_ctz(N);
+#pragma unroll
+  for (int p = 0; p < L / 3; ++p) {
+    stockham_pass<C, N, 8, TI>(buf, tw, Ns, inv);
+    Ns *= 8;
+  }
+  if constexpr (L % 3 == 2) stockham_pass<C, N, 4, TI>(buf, tw, Ns, inv);
+  if constexpr (L % 3 == 1) stockham_pass<C, N, 2, TI>(buf, tw, Ns, inv);
+}
+
+template <typename C, int N>
+__global__ __launch_bounds__(256) void k_poisson_z(C* __restrict__ spec, const double2* __restrict__ tw,
+                                                   const double* __restrict__ ktab, int nx, int ny,
+                                                   double inv_ntot)
+{
+#pragma clang fp contract(off)
+  constexpr int TI = 8;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf = reinterpret_cast<C*>(smem_raw);
+  const int nxh = nx / 2 + 1;
+  const int tiles = (nxh + TI - 1) / TI;
+  const int j = blockIdx.x / tiles, i0 = (blockIdx.x - j * tiles) * TI;
+  const long long S = (long long)nxh * ny;  // spectrum stride between z-planes
+  const long long base = (long long)j * nxh + i0;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < N * TI; e += 256) {
+    const int k = e / TI, col = e % TI;
+    C v;
+    v.x = 0;
+    v.y = 0;
+    if (i0 + col < nxh) v = spec[k * S + base + col];
+    buf[e] = v;
+  }
+  __syncthreads();
+  line_dft<C, N, TI>(buf, tw, false);
+  // the division of the spectral kernel, same operation order
+  const double* xk2 = ktab;
+  const double* tx = xk2 + nx;
+  const double* yk2 = tx + nx;
+  const double* ty = yk2 + ny;
+  const double* zk2 = ty + ny;
+  const double* tz = zk2 + N;
+  for (int e = tid; e < N * TI; e += 256) {
+    const int k = e / TI, col = e % TI, i = i0 + col;
+    if (i >= nxh) continue;
+    const double tyz = ty[j] * tz[k], txz = tx[i] * tz[k], txy = tx[i] * ty[j];
+    const double K = (xk2[i] * (tyz * tyz) + yk2[j] * (txz * txz)) + zk2[k] * (txy * txy);
+    C v = buf[e];
+    if (K < 1.0e-16) {
+      v.x = 0;
+      v.y = 0;
+    } else {
+      using R = typename CplxT<C>::R;
+      v.x = (R)(((double)v.x * inv_ntot) / (-K));
+      v.y = (R)(((double)v.y * inv_ntot) / (-K));
+    }
+    buf[e] = v;
+  }
+  __syncthreads();
+  line_dft<C, N, TI>(buf, tw, true);
+  for (int e = tid; e < N * TI; e += 256) {
+    const int k = e / TI, col = e % TI;
+    if (i0 + col < nxh) spec[k * S + base + col] = buf[e];
+  }
+}
+
+template <typename C>
+static int launch_poisson_z(x3d_ctx* c, C* spec)
+{
+  const int nxh = c->nx / 2 + 1;
+  const unsigned blocks = (unsigned)(((nxh + 7) / 8) * c->ny);
+  const size_t smem = (size_t)c->nz * 8 * sizeof(C);
+  const double inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
+  const double2* tw = reinterpret_cast<const double2*>(c->ztw);
+#define X3D_PZ(NN)                                                                                     \
+  case NN:                                                                                             \
+    if (smem > 65536)                                                                                  \
+      (void)hipFuncSetAttribute((const void*)k_poisson_z<C, NN>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)smem);                                                           \
+    hipLaunchKernelGGL((k_poisson_z<C, NN>), dim3(blocks), dim3(256), smem, c->stream, spec, tw, c->ktab, c->nx, \
+                       c->ny, inv_ntot);                                                              \
+    break;
+  switch (c->nz) {
+    X3D_PZ(8) X3D_PZ(16) X3D_PZ(32) X3D_PZ(64) X3D_PZ(128) X3D_PZ(256) X3D_PZ(512) X3D_PZ(1024)
+    default: return X3D_ERR_UNSUPPORTED;
+  }
+#undef X3D_PZ
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+#define X3D_TRY_RC(expr) do { const int _r = (expr); if (_r) return _r; } while (0)
+
+static bool zsplit_ok(int nz) { return nz >= 8 && nz <= 1024 && (nz & (nz - 1)) == 0; }
+
 // 1-D tables of abxyz (src/poisson.f90:436-476) and waves (:522-616):
 // modified wavenumber squared of the staggered derivative and the
 // interpolation transfer function, per axis, mirrored above n/2.
@@ -103,8 +321,26 @@ int poisson_setup(x3d_ctx* c)
   X3D_CHECK_HIP(hipMemcpy(c->ktab, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   const size_t ns = (size_t)(c->nx / 2 + 1) * c->ny * c->nz;
   X3D_CHECK_HIP(hipMalloc(&c->spec, ns * sizeof(double2)));
-  if (hipfftPlan3d(&c->fft_r2c, c->nz, c->ny, c->nx, HIPFFT_D2Z) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
-  if (hipfftPlan3d(&c->fft_c2r, c->nz, c->ny, c->nx, HIPFFT_Z2D) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  c->zsplit = zsplit_ok(c->nz);
+  if (c->zsplit) {
+    // 2-D transforms of every z-plane; the z stage is the fused kernel
+    std::vector<double2> tw(c->nz);
+    for (int t = 0; t < c->nz; ++t) {
+      const double a = -2.0 * PI * (double)t / (double)c->nz;
+      tw[t] = double2{std::cos(a), std::sin(a)};
+    }
+    X3D_CHECK_HIP(hipMalloc(&c->ztw, tw.size() * sizeof(double2)));
+    X3D_CHECK_HIP(hipMemcpy(c->ztw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+    int n2[2] = {c->ny, c->nx}, re[2] = {c->ny, c->nx}, ce[2] = {c->ny, c->nx / 2 + 1};
+    const int rd = c->nx * c->ny, cd = (c->nx / 2 + 1) * c->ny;
+    if (hipfftPlanMany(&c->fft_r2c, 2, n2, re, 1, rd, ce, 1, cd, HIPFFT_D2Z, c->nz) != HIPFFT_SUCCESS)
+      return X3D_ERR_FFT;
+    if (hipfftPlanMany(&c->fft_c2r, 2, n2, ce, 1, cd, re, 1, rd, HIPFFT_Z2D, c->nz) != HIPFFT_SUCCESS)
+      return X3D_ERR_FFT;
+  } else {
+    if (hipfftPlan3d(&c->fft_r2c, c->nz, c->ny, c->nx, HIPFFT_D2Z) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+    if (hipfftPlan3d(&c->fft_c2r, c->nz, c->ny, c->nx, HIPFFT_Z2D) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  }
   c->fft_ready = true;
   if (hipfftSetStream(c->fft_r2c, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (hipfftSetStream(c->fft_c2r, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
@@ -125,8 +361,10 @@ void poisson_release(x3d_ctx* c)
   }
   if (c->spec) (void)hipFree(c->spec);
   if (c->ktab) (void)hipFree(c->ktab);
+  if (c->ztw) (void)hipFree(c->ztw);
   c->spec = nullptr;
   c->ktab = nullptr;
+  c->ztw = nullptr;
 }
 
 // FP32 spectrum, FP64 division: the modified wavenumber and the quotient are
@@ -168,8 +406,16 @@ int poisson_prepare_f32(x3d_ctx* c)
 {
   if (!c->fft_ready) return X3D_ERR_FFT;
   if (!c->fft32_ready) {
-    if (hipfftPlan3d(&c->fft_r2c32, c->nz, c->ny, c->nx, HIPFFT_R2C) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
-    if (hipfftPlan3d(&c->fft_c2r32, c->nz, c->ny, c->nx, HIPFFT_C2R) != HIPFFT_SUCCESS) {
+    int n2[2] = {c->ny, c->nx}, re[2] = {c->ny, c->nx}, ce[2] = {c->ny, c->nx / 2 + 1};
+    const int rd = c->nx * c->ny, cd = (c->nx / 2 + 1) * c->ny;
+    const bool ok1 = c->zsplit ? hipfftPlanMany(&c->fft_r2c32, 2, n2, re, 1, rd, ce, 1, cd, HIPFFT_R2C,
+                                                c->nz) == HIPFFT_SUCCESS
+                               : hipfftPlan3d(&c->fft_r2c32, c->nz, c->ny, c->nx, HIPFFT_R2C) == HIPFFT_SUCCESS;
+    if (!ok1) return X3D_ERR_FFT;
+    const bool ok2 = c->zsplit ? hipfftPlanMany(&c->fft_c2r32, 2, n2, ce, 1, cd, re, 1, rd, HIPFFT_C2R,
+                                                c->nz) == HIPFFT_SUCCESS
+                               : hipfftPlan3d(&c->fft_c2r32, c->nz, c->ny, c->nx, HIPFFT_C2R) == HIPFFT_SUCCESS;
+    if (!ok2) {
       hipfftDestroy(c->fft_r2c32);
       return X3D_ERR_FFT;
     }
@@ -187,6 +433,12 @@ int poisson_solve_f32(x3d_ctx* c, float* rhs)
   // the FP64 spectrum buffer holds the FP32 one
   float2* spec = reinterpret_cast<float2*>(c->spec);
   if (hipfftExecR2C(c->fft_r2c32, rhs, (hipfftComplex*)spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  if (c->zsplit) {
+    const int rc2 = launch_poisson_z<float2>(c, spec);
+    if (rc2) return rc2;
+    if (hipfftExecC2R(c->fft_c2r32, (hipfftComplex*)spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+    return X3D_OK;
+  }
   const long long N = (long long)(c->nx / 2 + 1) * c->ny * c->nz;
   long long blocks = (N + 255) / 256;
   if (blocks > 65536) blocks = 65536;
@@ -202,6 +454,11 @@ int poisson_solve(x3d_ctx* c, double* rhs)
 {
   if (!c->fft_ready) return X3D_ERR_FFT;
   if (hipfftExecD2Z(c->fft_r2c, rhs, (hipfftDoubleComplex*)c->spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  if (c->zsplit) {
+    X3D_TRY_RC(launch_poisson_z<double2>(c, c->spec));
+    if (hipfftExecZ2D(c->fft_c2r, (hipfftDoubleComplex*)c->spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+    return X3D_OK;
+  }
   const long long N = (long long)(c->nx / 2 + 1) * c->ny * c->nz;
   long long blocks = (N + 255) / 256;
   if (blocks > 65536) blocks = 65536;
