@@ -139,16 +139,16 @@ __device__ __forceinline__ void small_dft(C (&v)[R], bool inv)
 
 // One Stockham pass of radix R over the TI lines held in buf[k][TI]
 // (Ns = product of the radices already applied).
-template <typename C, int N, int R, int TI>
+template <typename C, int N, int R, int TI, int NT>
 __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict__ tw, int Ns, bool inv)
 {
   constexpr int NB = TI * N / R;             // butterflies in the work-group
-  constexpr int BPT = (NB + 255) / 256;      // per thread
+  constexpr int BPT = (NB + NT - 1) / NT;    // per thread
   C v[BPT][R];
   const int tid = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
-    const int b = tid + 256 * q;
+    const int b = tid + NT * q;
     if (b < NB) {
       const int col = b % TI, jj = b / TI;
 #pragma unroll
@@ -158,13 +158,19 @@ __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict_
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < BPT; ++q) {
-    const int b = tid + 256 * q;
+    const int b = tid + NT * q;
     if (b < NB) {
       const int col = b % TI, jj = b / TI;
       const int kin = jj % Ns;
       const int step = N / (Ns * R);
+      // w^r from one table entry by repeated products (one load per butterfly)
+      const double2 w1 = tw[kin * step];
+      double2 wr = w1;
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(kin * r * step) & (N - 1)], inv);
+      for (int r = 1; r < R; ++r) {
+        v[q][r] = cmul(v[q][r], wr, inv);
+        if (r + 1 < R) wr = double2{wr.x * w1.x - wr.y * w1.y, wr.x * w1.y + wr.y * w1.x};
+      }
       small_dft<C, R>(v[q], inv);
       const int d = (jj / Ns) * Ns * R + kin;
 #pragma unroll
@@ -174,27 +180,27 @@ __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict_
   __syncthreads();
 }
 
-template <typename C, int N, int TI>
+template <typename C, int N, int TI, int NT>
 __device__ __forceinline__ void line_dft(C* buf, const double2* __restrict__ tw, bool inv)
 {
   int Ns = 1;
   constexpr int L = __builtin_ctz(N);
 #pragma unroll
   for (int p = 0; p < L / 3; ++p) {
-    stockham_pass<C, N, 8, TI>(buf, tw, Ns, inv);
+    stockham_pass<C, N, 8, TI, NT>(buf, tw, Ns, inv);
     Ns *= 8;
   }
-  if constexpr (L % 3 == 2) stockham_pass<C, N, 4, TI>(buf, tw, Ns, inv);
-  if constexpr (L % 3 == 1) stockham_pass<C, N, 2, TI>(buf, tw, Ns, inv);
+  if constexpr (L % 3 == 2) stockham_pass<C, N, 4, TI, NT>(buf, tw, Ns, inv);
+  if constexpr (L % 3 == 1) stockham_pass<C, N, 2, TI, NT>(buf, tw, Ns, inv);
 }
 
-template <typename C, int N>
-__global__ __launch_bounds__(256) void k_poisson_z(C* __restrict__ spec, const double2* __restrict__ tw,
-                                                   const double* __restrict__ ktab, int nx, int ny,
-                                                   double inv_ntot)
+// TI neighbouring x-columns per work-group of NT threads
+template <typename C, int N, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_poisson_z(C* __restrict__ spec, const double2* __restrict__ tw,
+                                                  const double* __restrict__ ktab, int nx, int ny,
+                                                  double inv_ntot)
 {
 #pragma clang fp contract(off)
-  constexpr int TI = 8;
   extern __shared__ unsigned char smem_raw[];
   C* buf = reinterpret_cast<C*>(smem_raw);
   const int nxh = nx / 2 + 1;
@@ -203,7 +209,7 @@ __global__ __launch_bounds__(256) void k_poisson_z(C* __restrict__ spec, const d
   const long long S = (long long)nxh * ny;  // spectrum stride between z-planes
   const long long base = (long long)j * nxh + i0;
   const int tid = threadIdx.x;
-  for (int e = tid; e < N * TI; e += 256) {
+  for (int e = tid; e < N * TI; e += NT) {
     const int k = e / TI, col = e % TI;
     C v;
     v.x = 0;
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(256) void k_poisson_z(C* __restrict__ spec, const d
     buf[e] = v;
   }
   __syncthreads();
-  line_dft<C, N, TI>(buf, tw, false);
+  line_dft<C, N, TI, NT>(buf, tw, false);
   // the division of the spectral kernel, same operation order
   const double* xk2 = ktab;
   const double* tx = xk2 + nx;
@@ -220,7 +226,7 @@ __global__ __launch_bounds__(256) void k_poisson_z(C* __restrict__ spec, const d
   const double* ty = yk2 + ny;
   const double* zk2 = ty + ny;
   const double* tz = zk2 + N;
-  for (int e = tid; e < N * TI; e += 256) {
+  for (int e = tid; e < N * TI; e += NT) {
     const int k = e / TI, col = e % TI, i = i0 + col;
     if (i >= nxh) continue;
     const double tyz = ty[j] * tz[k], txz = tx[i] * tz[k], txy = tx[i] * ty[j];
@@ -230,43 +236,65 @@ __global__ __launch_bounds__(256) void k_poisson_z(C* __restrict__ spec, const d
       v.x = 0;
       v.y = 0;
     } else {
+      // one division per mode (the separate kernel divides each part)
       using R = typename CplxT<C>::R;
-      v.x = (R)(((double)v.x * inv_ntot) / (-K));
-      v.y = (R)(((double)v.y * inv_ntot) / (-K));
+      const double q = inv_ntot / (-K);
+      v.x = (R)((double)v.x * q);
+      v.y = (R)((double)v.y * q);
     }
     buf[e] = v;
   }
   __syncthreads();
-  line_dft<C, N, TI>(buf, tw, true);
-  for (int e = tid; e < N * TI; e += 256) {
+  line_dft<C, N, TI, NT>(buf, tw, true);
+  for (int e = tid; e < N * TI; e += NT) {
     const int k = e / TI, col = e % TI;
     if (i0 + col < nxh) spec[k * S + base + col] = buf[e];
   }
 }
 
+template <typename C, int N, int TI, int NT>
+static int launch_pz(x3d_ctx* c, C* spec)
+{
+  const int nxh = c->nx / 2 + 1;
+  const unsigned blocks = (unsigned)(((nxh + TI - 1) / TI) * c->ny);
+  const size_t smem = (size_t)N * TI * sizeof(C);
+  const double inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
+  const double2* tw = reinterpret_cast<const double2*>(c->ztw);
+  if (smem > 65536)
+    (void)hipFuncSetAttribute((const void*)k_poisson_z<C, N, TI, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)smem);
+  hipLaunchKernelGGL((k_poisson_z<C, N, TI, NT>), dim3(blocks), dim3(NT), smem, c->stream, spec, tw, c->ktab,
+                     c->nx, c->ny, inv_ntot);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+// Tile (measured at 512^3): FP64 16 columns (256-byte rows) where 16 lines fit
+// in 128 KiB of LDS, else 8; FP32 8 columns (its 16-column tile needs more
+// registers than it saves in row length).  512 threads for tiles above
+// 64 KiB (one work-group per compute unit), 256 otherwise.
+template <typename C, int N>
+static int launch_pz_n(x3d_ctx* c, C* spec)
+{
+  constexpr int TI = (sizeof(C) == 16 && (size_t)N * 16 * sizeof(C) <= 131072) ? 16 : 8;
+  constexpr int NT = (size_t)N * TI * sizeof(C) > 65536 ? 512 : 256;
+  return launch_pz<C, N, TI, NT>(c, spec);
+}
+
 template <typename C>
 static int launch_poisson_z(x3d_ctx* c, C* spec)
 {
-  const int nxh = c->nx / 2 + 1;
-  const unsigned blocks = (unsigned)(((nxh + 7) / 8) * c->ny);
-  const size_t smem = (size_t)c->nz * 8 * sizeof(C);
-  const double inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
-  const double2* tw = reinterpret_cast<const double2*>(c->ztw);
-#define X3D_PZ(NN)                                                                                     \
-  case NN:                                                                                             \
-    if (smem > 65536)                                                                                  \
-      (void)hipFuncSetAttribute((const void*)k_poisson_z<C, NN>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                (int)smem);                                                           \
-    hipLaunchKernelGGL((k_poisson_z<C, NN>), dim3(blocks), dim3(256), smem, c->stream, spec, tw, c->ktab, c->nx, \
-                       c->ny, inv_ntot);                                                              \
-    break;
   switch (c->nz) {
-    X3D_PZ(8) X3D_PZ(16) X3D_PZ(32) X3D_PZ(64) X3D_PZ(128) X3D_PZ(256) X3D_PZ(512) X3D_PZ(1024)
+    case 8: return launch_pz_n<C, 8>(c, spec);
+    case 16: return launch_pz_n<C, 16>(c, spec);
+    case 32: return launch_pz_n<C, 32>(c, spec);
+    case 64: return launch_pz_n<C, 64>(c, spec);
+    case 128: return launch_pz_n<C, 128>(c, spec);
+    case 256: return launch_pz_n<C, 256>(c, spec);
+    case 512: return launch_pz_n<C, 512>(c, spec);
+    case 1024: return launch_pz_n<C, 1024>(c, spec);
     default: return X3D_ERR_UNSUPPORTED;
   }
-#undef X3D_PZ
-  X3D_CHECK_HIP(hipGetLastError());
-  return X3D_OK;
 }
 
 #define X3D_TRY_RC(expr) do { const int _r = (expr); if (_r) return _r; } while (0)
