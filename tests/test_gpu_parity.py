@@ -4,8 +4,9 @@ Inputs are seeded and identical on both sides (built on the host, uploaded).
 Floating-point tolerance: 1e-12 relative in the max norm, the FP64 bar for the
 derivatives and the timestep; the transposes are compared bit for bit.
 Grids are chosen to exercise every line tiling: chunk lengths M = 1..5,
-chunks per line C = 8..64, lines per wavefront 1..16, and three different
-domain lengths so that an axis mix-up cannot cancel.
+chunks per line C = 8..64, lines per wavefront 1..16, odd and even x chunks
+(even ones move two points per access), and different domain lengths so
+that an axis mix-up cannot cancel.
 """
 import numpy as np
 import pytest
@@ -19,6 +20,7 @@ GRIDS = [
     (64, 48, 32, 1.0, 2.0, 3.0),
     (40, 96, 160, 2 * np.pi, 1.0, 0.5),
     (16, 8, 24, 1.0, 1.0, 1.0),
+    (128, 24, 16, 1.0, 1.0, 2.0),
 ]
 
 

@@ -14,7 +14,8 @@ these grids in brackets):
     solenoidal field (Taylor-Green + 1e-3 noise), a sum of O(1) terms that
     cancel to O(1e-3), so FP32 keeps about three fewer digits there.
 The grids cover every FP32 chunk length the plans use at these sizes,
-including 24- and 32-point chunks (lines of 768 and 512 points).
+including 24- and 32-point chunks (lines of 768 and 512 points) and even x
+chunks (two points per access).
 """
 import numpy as np
 import pytest
@@ -29,6 +30,8 @@ GRIDS = [
     (16, 8, 24, 1.0, 1.0, 1.0),
     (24, 512, 8, 1.0, 2 * np.pi, 1.0),
     (16, 8, 768, 1.0, 1.0, 2 * np.pi),
+    (128, 24, 16, 1.0, 1.0, 2.0),
+    (512, 8, 16, 2 * np.pi, 1.0, 1.0),
 ]
 
 

@@ -281,19 +281,67 @@ __device__ __forceinline__ XBlock<T, M> x_block(const Geom& geo, const LineThrea
   return b;
 }
 
+// Two consecutive points in one buffer access (16 B for FP64, 8 B for FP32).
+// The loaded vector is reinterpreted as a whole: with ROCm 7.2, extracting
+// the 32-bit elements of a raw_buffer_load_b64 result one by one and
+// bit-casting each to float compiled to a single-dword load whose value was
+// used for both points.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef double f64x2v __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ void ld2(const T* base, uint32_t boff, T& a, T& b)
+{
+  if constexpr (sizeof(T) == 8) {
+    const f64x2v v = __builtin_bit_cast(f64x2v, __builtin_amdgcn_raw_buffer_load_b128(field_rsrc(base), boff, 0, 0));
+    a = v.x;
+    b = v.y;
+  } else {
+    const f32x2v v = __builtin_bit_cast(f32x2v, __builtin_amdgcn_raw_buffer_load_b64(field_rsrc(base), boff, 0, 0));
+    a = v.x;
+    b = v.y;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void st2(T* base, uint32_t boff, T a, T b)
+{
+  if constexpr (sizeof(T) == 8) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, f64x2v{a, b}), field_rsrc(base), boff, 0, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, f32x2v{a, b}), field_rsrc(base), boff, 0, 0);
+  }
+}
+
 // Coalesced load of the block into the tile (point e of the block goes to
-// row e / M, column e % M).
+// row e / M, column e % M).  With M even each lane moves two neighbouring
+// points per access (both in one chunk); lines have even length then, so
+// pairs never straddle the end of the field.
 template <typename T, int M>
 __device__ __forceinline__ void x_stage_in(const T* __restrict__ in, const XBlock<T, M>& b)
 {
   constexpr int P = XPitch<M>::P;
   wave_lds_sync();
+  if constexpr (M % 2 == 0) {
+    const uint32_t last = b.lim - (uint32_t)sizeof(T);  // first point of the field's last pair
 #pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const int e = b.lane + 64 * k;
-    uint32_t off = b.base + (uint32_t)e * (uint32_t)sizeof(T);
-    off = off < b.lim ? off : b.lim;
-    b.tile[(e / M) * P + e % M] = ld(in, off);
+    for (int k = 0; k < M / 2; ++k) {
+      const int e = 2 * b.lane + 128 * k;
+      uint32_t off = b.base + (uint32_t)e * (uint32_t)sizeof(T);
+      off = off < last ? off : last;
+      T v0, v1;
+      ld2(in, off, v0, v1);
+      T* dst = b.tile + (e / M) * P + e % M;
+      dst[0] = v0;
+      dst[1] = v1;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = b.lane + 64 * k;
+      uint32_t off = b.base + (uint32_t)e * (uint32_t)sizeof(T);
+      off = off < b.lim ? off : b.lim;
+      b.tile[(e / M) * P + e % M] = ld(in, off);
+    }
   }
   wave_lds_sync();
 }
@@ -349,10 +397,19 @@ __device__ __forceinline__ void x_store(T* out, const XBlock<T, M>& b, const T (
 #pragma unroll
   for (int k = 0; k < M; ++k) own[k] = v[k];
   wave_lds_sync();
+  if constexpr (M % 2 == 0) {
 #pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const int e = b.lane + 64 * k;
-    if (e < b.valid) st(out, b.base + (uint32_t)e * (uint32_t)sizeof(T), b.tile[(e / M) * P + e % M]);
+    for (int k = 0; k < M / 2; ++k) {
+      const int e = 2 * b.lane + 128 * k;
+      const T* src = b.tile + (e / M) * P + e % M;
+      if (e < b.valid) st2(out, b.base + (uint32_t)e * (uint32_t)sizeof(T), src[0], src[1]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = b.lane + 64 * k;
+      if (e < b.valid) st(out, b.base + (uint32_t)e * (uint32_t)sizeof(T), b.tile[(e / M) * P + e % M]);
+    }
   }
 }
 
