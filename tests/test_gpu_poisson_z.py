@@ -37,3 +37,30 @@ def test_poisson_fused_z(torch_cuda, nz):
     ctx.close()
     assert e64 < 1e-12, e64
     assert e32 < 2e-6, e32
+
+
+@pytest.mark.parametrize("ny", [8, 32, 128, 512, 1024])
+def test_poisson_fused_z_y_lengths(torch_cuda, ny):
+    """The fused z stage under 2-D plans of several y lengths (the 2-D
+    hipFFT layout of the half spectrum that the z kernel reads)."""
+    import torch
+    from x3div_amd import X3D
+    nx, nz = (20, 16) if ny <= 128 else (10, 8)
+    print(f"\n[poisson y {nx}x{ny}x{nz}]", flush=True)
+    ctx = X3D(nx, ny, nz, 2.0, 1.0, 3.0)
+    orc = Oracle(nx, ny, nz, 2.0, 1.0, 3.0)
+    rhs = np.random.default_rng(ny).standard_normal(orc.shape)
+    ref = orc.poisson(rhs)
+    t = torch.from_numpy(rhs.copy()).cuda()
+    ctx.poisson(t)
+    r32 = np.asarray(rhs, dtype=np.float32)
+    ref32 = orc.poisson(r32.astype(np.float64))
+    t32 = torch.from_numpy(r32.copy()).cuda()
+    ctx.poisson_f32(t32)
+    torch.cuda.synchronize()
+    e64 = np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max()
+    e32 = np.abs(t32.cpu().numpy().astype(np.float64) - ref32).max() / np.abs(ref32).max()
+    print(f"ny={ny}: fp64 {e64:.2e} fp32 {e32:.2e}")
+    ctx.close()
+    assert e64 < 1e-12, e64
+    assert e32 < 2e-6, e32
