@@ -252,8 +252,8 @@ int x3d_batch_step(x3d_batch *batch, double gdt);
 /* Record one batched step into a HIP graph (gdt fixed at capture); replay it. */
 int x3d_batch_graph_capture(x3d_batch *batch, double gdt);
 int x3d_batch_graph_launch(x3d_batch *batch);
-/* Time the batched step (reps steps per trial) for every y/z tiling that fits
- * each line length, keep the fastest, and write a text report. */
+/* Time the batched step (reps steps per trial) for every tiling that fits
+ * each direction's line lengths, keep the fastest, and write a text report. */
 int x3d_batch_autotune(x3d_batch *batch, int reps, char *report, size_t report_len);
 
 /* ---- test mode and fields --------------------------------------------- */

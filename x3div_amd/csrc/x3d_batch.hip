@@ -3,8 +3,8 @@
 // (src/xcompact3d.f90:80-95 on its own nx, ny, nz); the batch owns their
 // fields, groups the grids by line tiling so that each step pass is one
 // persistent launch per group, and can record the whole batched step in a
-// HIP graph.  An autotuner picks the y/z tiling of each line length by timing
-// the batched step.
+// HIP graph.  An autotuner picks the tiling of each direction and line length
+// by timing the batched step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,6 +74,31 @@ size_t plan_lds_bytes(const LinePlan& p, int prec)
   const size_t lpw = (size_t)p.Wl * p.LW, threads = 64u * p.Wc * p.Wl;
   const size_t solver = p.Wc > 1 ? 4 * p.Wc * lpw + 2 * lpw : 0;
   return (size_t)prec * (solver + threads * (size_t)(2 * p.M + 4));
+}
+
+// x-line tilings: chunks of M points, C = n / M chunks per line (a power of
+// two up to 64, so that a wavefront holds 64 / C whole lines), Wl wavefronts
+// per work-group, each with its LDS staging tile.
+std::vector<LinePlan> candidates_x(int n, int prec, size_t lds_max)
+{
+  static const int ms64[] = {1, 2, 3, 4, 5, 6, 8, 12, 16};
+  static const int ms32[] = {1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32};
+  std::vector<LinePlan> out;
+  const int* ms = prec == 8 ? ms64 : ms32;
+  const int nm = prec == 8 ? 9 : 11;
+  for (int k = 0; k < nm; ++k) {
+    const int m = ms[k];
+    if (n % m) continue;
+    const int c = n / m;
+    if (c > 64 || (c & (c - 1))) continue;
+    for (int wl : {1, 2, 4, 8}) {
+      LinePlan p;
+      p.n = n; p.M = m; p.C = c; p.G = c; p.Wc = 1; p.Wl = wl; p.LW = 64 / c; p.line_fast = false;
+      const size_t pitch = (m & 1) ? m : m + 1;
+      if ((size_t)prec * wl * 64 * pitch <= lds_max) out.push_back(p);
+    }
+  }
+  return out;
 }
 
 std::vector<LinePlan> candidates(int n, int prec, size_t lds_max)
@@ -299,14 +324,16 @@ int x3d_batch_autotune(x3d_batch* b, int reps, char* report, size_t report_len)
   log += line;
   // coordinate descent: one (direction, line length) at a time, every grid of
   // that length switched together
-  for (int dir = 1; dir <= 2; ++dir) {
+  for (int dir = 0; dir <= 2; ++dir) {
     std::map<int, std::vector<int>> byn;
-    for (int i = 0; i < b->ng; ++i) byn[dir == 1 ? b->g[i]->ny : b->g[i]->nz].push_back(i);
+    for (int i = 0; i < b->ng; ++i) byn[dir == 0 ? b->g[i]->nx : dir == 1 ? b->g[i]->ny : b->g[i]->nz].push_back(i);
     for (auto& kv : byn) {
       auto plan_of = [&](int i) -> LinePlan& { return b->prec == 8 ? b->g[i]->fplan[dir] : b->g[i]->fplan32[dir]; };
       const LinePlan keep = plan_of(kv.second[0]);
       LinePlan best = keep;
-      for (const LinePlan& cand : candidates(kv.first, b->prec, lds_max)) {
+      const std::vector<LinePlan> cands =
+          dir == 0 ? candidates_x(kv.first, b->prec, lds_max) : candidates(kv.first, b->prec, lds_max);
+      for (const LinePlan& cand : cands) {
         if (cand.M == keep.M && cand.G == keep.G && cand.Wc == keep.Wc && cand.Wl == keep.Wl) continue;
         for (int i : kv.second) plan_of(i) = cand;
         b->built = false;
@@ -321,7 +348,7 @@ int x3d_batch_autotune(x3d_batch* b, int reps, char* report, size_t report_len)
       for (int i : kv.second) plan_of(i) = best;
       b->built = false;
       if (rc) return rc;
-      std::snprintf(line, sizeof line, "%c n=%d: M=%d G=%d Wc=%d Wl=%d -> %.4f ms/step\n", dir == 1 ? 'y' : 'z',
+      std::snprintf(line, sizeof line, "%c n=%d: M=%d G=%d Wc=%d Wl=%d -> %.4f ms/step\n", "xyz"[dir],
                     kv.first, best.M, best.G, best.Wc, best.Wl, best_ms);
       log += line;
     }
