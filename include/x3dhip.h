@@ -203,7 +203,7 @@ int x3d_poisson_f32(x3d_ctx *ctx, float *rhs);
 /* Matrix-core variant of x3d_apply_f32: with periodic boundaries each
  * operator is the dense circulant C = A^-1 B (built once per operator in
  * FP64 from the reference's own periodic solve), applied to every line as a
- * GEMM on v_mfma_f32_32x32x8_f16 in three split-precision terms
+ * GEMM on v_mfma_f32_32x32x16_f16 in three split-precision terms
  * (C_hi u_hi + C_hi u_lo + C_lo u_hi, fp16 halves after exact power-of-two
  * scaling, FP32 accumulation); a few 1e-6 relative to the FP64 operator.  Needs the line length a
  * multiple of 128 (at most 1024) and the line count a multiple of 32;

@@ -1,6 +1,6 @@
 """The dense-circulant matrix-core variant against the FP64 oracle.
 
-Every periodic operator is applied as C = A^-1 B on v_mfma_f32_32x32x8_f16
+Every periodic operator is applied as C = A^-1 B on v_mfma_f32_32x32x16_f16
 with three split-precision (fp16 hi/lo) terms and FP32 accumulation.  The
 dropped lo*lo term and the fp16 rounding of the lo halves are each about
 2^-22 relative per product; a derivative of a smooth field cancels terms of

@@ -7,7 +7,7 @@
 // with K = n:
 //   y/z lines:  OUT[i][col] = sum_k C[i][k] IN[k][col]   (IN viewed as n x N)
 //   x lines:    OUT[l][i]   = sum_k IN[l][k] C[i][k]     (IN viewed as L x n)
-// computed with v_mfma_f32_32x32x8_f16 in three split-precision terms,
+// computed with v_mfma_f32_32x32x16_f16 in three split-precision terms,
 // x = x_hi + x_lo (fp16 each, together ~22 bits):
 //   C_hi IN_hi + C_hi IN_lo + C_lo IN_hi, accumulated in FP32.
 // fp16 keeps three more bits per half than bf16 but has a narrow exponent
@@ -25,25 +25,25 @@
 
 namespace x3d {
 
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// hi/lo fp16 halves of four FP32 values
-__device__ __forceinline__ void split4(const float (&x)[4], f16x4& hi, f16x4& lo)
+// hi/lo fp16 halves of eight FP32 values
+__device__ __forceinline__ void split8(const float (&x)[8], f16x8& hi, f16x8& lo)
 {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < 8; ++v) {
     const _Float16 h = (_Float16)x[v];
     hi[v] = h;
     lo[v] = (_Float16)(x[v] - (float)h);
   }
 }
 
-// One fragment of C for the 32x32x8 operand layout (lane l: row l % 32,
-// k = 4 (l / 32) + v): hi and lo halves of 4 values, indexed by the block
-// offset d = (row0 - k0) mod n, d a multiple of 8.
+// One fragment of C for the 32x32x16 operand layout (lane l: row l % 32,
+// k = 8 (l / 32) + v): hi and lo halves of 8 values, indexed by the block
+// offset d = (row0 - k0) mod n, d a multiple of 16.
 struct CircFrag {
-  f16x4 hi, lo;
+  f16x8 hi, lo;
 };
 
 // y/z: OUT = C IN, IN rows k at stride rs, WG columns [col0, col0 + 32) of
@@ -56,8 +56,13 @@ __global__ __launch_bounds__(256) void k_circ_mfma(const float* __restrict__ in,
                                                    long long ss, float inv_cscale)
 {
   __shared__ float wmax[4];
+  // the operator's fragments (n / 16 x 64 lanes x 32 B) are read by every
+  // wavefront at every step: staged once per work-group in LDS
+  extern __shared__ unsigned char smem_raw[];
+  CircFrag* ltab = reinterpret_cast<CircFrag*>(smem_raw);
+  for (int q = threadIdx.x; q < (n >> 4) * 64; q += 256) ltab[q] = tab[q];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lc = lane & 31, lk = 4 * (lane >> 5);
+  const int lc = lane & 31, lk8 = 8 * (lane >> 5), lk = 4 * (lane >> 5);
   long long base;
   if constexpr (XDIR) base = ((long long)blockIdx.x * 32 + lc) * n;  // this lane's line
   else base = (long long)blockIdx.y * ss + (long long)blockIdx.x * 32 + lc;  // this lane's column
@@ -83,33 +88,33 @@ __global__ __launch_bounds__(256) void k_circ_mfma(const float* __restrict__ in,
   for (int r = 0; r < RB; ++r)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[r][v] = 0.f;
-  for (int k0 = 0; k0 < n; k0 += 8) {
-    float x[4];
+  for (int k0 = 0; k0 < n; k0 += 16) {
+    float x[8];
     if constexpr (XDIR) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) x[v] = in[base + k0 + lk + v] * scale;
+      for (int v = 0; v < 8; ++v) x[v] = in[base + k0 + lk8 + v] * scale;
     } else {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) x[v] = in[base + (long long)(k0 + lk + v) * rs] * scale;
+      for (int v = 0; v < 8; ++v) x[v] = in[base + (long long)(k0 + lk8 + v) * rs] * scale;
     }
-    f16x4 fh, fl;
-    split4(x, fh, fl);
+    f16x8 fh, fl;
+    split8(x, fh, fl);
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const int b0 = 32 * (wave * RB + r);
       int d = b0 - k0;
       d = d < 0 ? d + n : d;
-      const CircFrag cf = tab[(d >> 3) * 64 + lane];
+      const CircFrag cf = ltab[(d >> 4) * 64 + lane];
       if constexpr (XDIR) {
         // D[line][i] = sum_k IN[line][k] C^T[k][i]
-        acc[r] = __builtin_amdgcn_mfma_f32_32x32x8f16(fh, cf.hi, acc[r], 0, 0, 0);
-        acc[r] = __builtin_amdgcn_mfma_f32_32x32x8f16(fl, cf.hi, acc[r], 0, 0, 0);
-        acc[r] = __builtin_amdgcn_mfma_f32_32x32x8f16(fh, cf.lo, acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fh, cf.hi, acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fl, cf.hi, acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fh, cf.lo, acc[r], 0, 0, 0);
       } else {
         // D[i][col] = sum_k C[i][k] IN[k][col]
-        acc[r] = __builtin_amdgcn_mfma_f32_32x32x8f16(cf.hi, fh, acc[r], 0, 0, 0);
-        acc[r] = __builtin_amdgcn_mfma_f32_32x32x8f16(cf.hi, fl, acc[r], 0, 0, 0);
-        acc[r] = __builtin_amdgcn_mfma_f32_32x32x8f16(cf.lo, fh, acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cf.hi, fh, acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cf.hi, fl, acc[r], 0, 0, 0);
+        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cf.lo, fh, acc[r], 0, 0, 0);
       }
     }
   }
@@ -189,19 +194,19 @@ extern "C" int x3d_apply_circ_mfma(x3d_ctx* c, int op, float* out, const float* 
     if (cmax > 0.0) (void)std::frexp(cmax, &ce);
     const double cs = std::ldexp(1.0, 10 - ce);
     c->circ_scale[op] = (float)std::ldexp(1.0, ce - 10);
-    std::vector<CircFrag> h((size_t)(n / 8) * 64);
-    for (int d = 0; d < n; d += 8)
+    std::vector<CircFrag> h((size_t)(n / 16) * 64);
+    for (int d = 0; d < n; d += 16)
       for (int l = 0; l < 64; ++l) {
         CircFrag f;
-        for (int v = 0; v < 4; ++v) {
-          int idx = (d + (l & 31) - 4 * (l >> 5) - v) % n;
+        for (int v = 0; v < 8; ++v) {
+          int idx = (d + (l & 31) - 8 * (l >> 5) - v) % n;
           if (idx < 0) idx += n;
           const double x = col[idx] * cs;
           const _Float16 hi = (_Float16)x;
           f.hi[v] = hi;
           f.lo[v] = (_Float16)(x - (double)hi);
         }
-        h[(size_t)(d / 8) * 64 + l] = f;
+        h[(size_t)(d / 16) * 64 + l] = f;
       }
     void* p = nullptr;
     X3D_CHECK_HIP(hipMalloc(&p, h.size() * sizeof(CircFrag)));
@@ -228,12 +233,19 @@ extern "C" int x3d_apply_circ_mfma(x3d_ctx* c, int op, float* out, const float* 
   }
   const int RB = n / 128;
   const float inv = c->circ_scale[op];  // undoes the host scaling of C
+  const size_t smem = (size_t)(n / 16) * 64 * sizeof(CircFrag);  // 128 n bytes
 #define X3D_RB(R)                                                                                         \
   case R:                                                                                                 \
+    if (smem > 65536) {                                                                                   \
+      (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, true>,                                        \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);                   \
+      (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, false>,                                       \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);                   \
+    }                                                                                                     \
     if (dir == 0)                                                                                         \
-      hipLaunchKernelGGL((k_circ_mfma<R, true>), grid, dim3(256), 0, c->stream, in, out, tab, n, rs, ss, inv); \
+      hipLaunchKernelGGL((k_circ_mfma<R, true>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_circ_mfma<R, false>), grid, dim3(256), 0, c->stream, in, out, tab, n, rs, ss, inv); \
+      hipLaunchKernelGGL((k_circ_mfma<R, false>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
     break;
   switch (RB) {
     X3D_RB(1) X3D_RB(2) X3D_RB(3) X3D_RB(4) X3D_RB(5) X3D_RB(6) X3D_RB(7) X3D_RB(8)
