@@ -269,15 +269,15 @@ static int launch_pz(x3d_ctx* c, C* spec)
   return X3D_OK;
 }
 
-// Tile (measured at 512^3): FP64 16 columns (256-byte rows) where 16 lines fit
-// in 128 KiB of LDS, else 8; FP32 8 columns (its 16-column tile needs more
-// registers than it saves in row length).  512 threads for tiles above
-// 64 KiB (one work-group per compute unit), 256 otherwise.
+// Tile (measured at 512^3): 16 columns where 16 lines fit in 128 KiB of LDS,
+// else 8; 1024 threads for tiles of 64 KiB and more (the work-group is then
+// the compute unit's only one or one of two, and its own waves hide the
+// latency), 256 otherwise.
 template <typename C, int N>
 static int launch_pz_n(x3d_ctx* c, C* spec)
 {
-  constexpr int TI = (sizeof(C) == 16 && (size_t)N * 16 * sizeof(C) <= 131072) ? 16 : 8;
-  constexpr int NT = (size_t)N * TI * sizeof(C) > 65536 ? 512 : 256;
+  constexpr int TI = ((size_t)N * 16 * sizeof(C) <= 131072) ? 16 : 8;
+  constexpr int NT = (size_t)N * TI * sizeof(C) >= 65536 ? 1024 : 256;
   return launch_pz<C, N, TI, NT>(c, spec);
 }
 
