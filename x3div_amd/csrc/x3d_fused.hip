@@ -47,8 +47,14 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
   const LineThread t = line_thread<T, DIR>(geo, A.tl, M, blk);
   const int n = geo.n;
   T* const Ud = DIR == 0 ? ux : (DIR == 1 ? uy : uz);
-  XBlock<T, M> xb;
-  if constexpr (DIR == 0) xb = x_block<T, M>(geo, t, lds);
+  // x: a second per-wavefront tile holds u_x (= u_d) for the whole kernel,
+  // staged once; the first tile stages each u_c
+  XBlock<T, M> xb, xd;
+  if constexpr (DIR == 0) {
+    xb = x_block<T, M>(geo, t, lds);
+    xd = x_block<T, M>(geo, t, lds + (blockDim.x >> 6) * x_tile_words<M>());
+    x_stage_in<T, M>(Ud, xd);
+  }
   // y/z: the u_d window (the same for the three components) and, with the
   // Euler update, each u_c chunk are kept in per-thread LDS slots from their
   // loads to their later uses instead of being loaded again: strided loads
@@ -70,8 +76,13 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
     {
       T wc[NW], wd[NW];
       if constexpr (DIR == 0) {
-        x_load_window<T, M, H, H>(Uc, xb, t, wc);
-        x_load_window<T, M, H, H>(Ud, xb, t, wd);
+        x_window<T, M, H, H>(xd, t, wd);
+        if (c == 0) {
+#pragma unroll
+          for (int q = 0; q < NW; ++q) wc[q] = wd[q];
+        } else {
+          x_load_window<T, M, H, H>(Uc, xb, t, wc);
+        }
       } else {
         if (c == 0) {
           load_window<T, M, H, H, true>(Ud, t, n, wd);
@@ -108,8 +119,7 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
     __builtin_amdgcn_sched_barrier(0);
     // u_d at the chunk points again (a cache hit) rather than kept live
     if constexpr (DIR == 0) {
-      T ud[M];
-      x_load_chunk<T, M>(Ud, xb, ud);
+      const T* ud = xd.tile + xd.lane * XPitch<M>::P;
 #pragma unroll
       for (int k = 0; k < M; ++k) v1[k] = v1[k] + ud[k] * v2[k];
     } else {
@@ -406,6 +416,8 @@ static DirSet<T> dirset(const x3d_ctx* c, int dir)
   s.smem_tq = s.smem;
   if (P.line_fast)
     s.smem_tq += sizeof(T) * (size_t)s.block.x * (size_t)(P.M + 4 + (dir == 2 ? P.M : 0));
+  else
+    s.smem_tq = 2 * s.smem;  // x: the staging tile and the u_x tile
   const Scheme& sc = c->sc[dir];
   // coefficients in FP64, rounded once to T
   auto scaled = [](Coef<double> k, double alfa) {
