@@ -2,7 +2,13 @@
 forward z DFT, the division and the inverse z DFT) against the oracle, for
 every power-of-two z length the kernel is built for (radix-8 passes with a
 final radix-4 or radix-2 pass), in both precisions.  FP64 at the 1e-12 bar,
-FP32 at 2e-6 (the precision form's Poisson bound)."""
+FP32 at 2e-6 (the precision form's Poisson bound).
+
+Lines of up to 512 points run the persistent, pipelined z kernel (each
+work-group walks several tiles and prefetches the next one); X3D_PZ_PLAIN
+selects the one-tile-per-work-group kernel.  Both do the same arithmetic in
+the same order, so on grids with many tiles per work-group their results
+must agree bit for bit."""
 import numpy as np
 import pytest
 
@@ -64,3 +70,36 @@ def test_poisson_fused_z_y_lengths(torch_cuda, ny):
     ctx.close()
     assert e64 < 1e-12, e64
     assert e32 < 2e-6, e32
+
+
+@pytest.mark.parametrize("shape", [(512, 256, 64), (64, 96, 512)])
+def test_poisson_pipelined_z_many_tiles(torch_cuda, shape, monkeypatch):
+    """Grids with more tiles than resident work-groups: the pipelined kernel
+    against the oracle (FP64) and bit for bit against the plain kernel."""
+    import torch
+    from x3div_amd import X3D
+    nx, ny, nz = shape
+    print(f"\n[poisson pipelined {nx}x{ny}x{nz}]", flush=True)
+    ctx = X3D(nx, ny, nz, 1.0, 2.0, 3.0)
+    rhs = np.random.default_rng(nx + nz).standard_normal((nz, ny, nx))
+    outs = {}
+    for plain in (False, True):
+        if plain:
+            monkeypatch.setenv("X3D_PZ_PLAIN", "1")
+        else:
+            monkeypatch.delenv("X3D_PZ_PLAIN", raising=False)
+        t = torch.from_numpy(rhs.copy()).cuda()
+        ctx.poisson(t)
+        t32 = torch.from_numpy(rhs.astype(np.float32)).cuda()
+        ctx.poisson_f32(t32)
+        torch.cuda.synchronize()
+        outs[plain] = (t.cpu().numpy(), t32.cpu().numpy())
+    monkeypatch.delenv("X3D_PZ_PLAIN", raising=False)
+    ctx.close()
+    assert np.array_equal(outs[False][0], outs[True][0])
+    assert np.array_equal(outs[False][1], outs[True][1])
+    orc = Oracle(nx, ny, nz, 1.0, 2.0, 3.0)
+    ref = orc.poisson(rhs)
+    e64 = np.abs(outs[False][0] - ref).max() / np.abs(ref).max()
+    print(f"{shape}: fp64 {e64:.2e}")
+    assert e64 < 1e-12, e64

@@ -111,6 +111,19 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
                 cf.b * (wd[q + 2] * wc[q + 2] - wd[q - 2] * wc[q - 2]);
       }
     }
+    // y/z: the epilogue's strided loads (du_c from the x pass, resp. tmp_c)
+    // are issued here and complete during the two line solves, whose
+    // barriers wait on LDS only.  Neither field is written by this pass.
+    // (not for FP32 z-lines of 32-point chunks, which would spill)
+    constexpr bool PF = DIR != 0 && !(sizeof(T) == 4 && M >= 32 && DIR == 2);
+    T a[M];
+    if constexpr (PF) {
+      if (t.active) {
+        const T* src = DIR == 1 ? DUc : TMc;
+#pragma unroll
+        for (int k = 0; k < M; ++k) a[k] = ld(src, t.at(t.j0 + k));
+      }
+    }
     // keep the two line solves apart: interleaving them doubles the live state
     __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v2, t, cc, lds);
@@ -131,20 +144,16 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
     } else if constexpr (DIR == 1) {
       // tmp_c = du_c (x part) + r_c: the sum is formed here, where a strided
       // row stays inside one page, rather than in the z pass
-      // (all loads before the first store: the stores may alias as far as
-      // the compiler knows, and would otherwise serialise the loads)
       if (t.active) {
-        T a[M];
-#pragma unroll
-        for (int k = 0; k < M; ++k) a[k] = ld(DUc, t.at(t.j0 + k));
 #pragma unroll
         for (int k = 0; k < M; ++k) st(TMc, t.at(t.j0 + k), a[k] + v1[k]);
       }
     } else {
       if (t.active) {
-        T a[M];
+        if constexpr (!PF) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) a[k] = ld(TMc, t.at(t.j0 + k));
+          for (int k = 0; k < M; ++k) a[k] = ld(TMc, t.at(t.j0 + k));
+        }
 #pragma unroll
         for (int k = 0; k < M; ++k) {
           const uint32_t at = t.at(t.j0 + k);
@@ -214,27 +223,30 @@ __device__ __forceinline__ void div_body(const PassArgs<T>& __restrict__ A, int 
     solve_circ<T, M, false>(v, t, cci, lds);
     x_store<T, M>(o3, xb, v);
   } else if constexpr (DIR == 1) {
+    // each input window is loaded before the preceding line solve, so that
+    // its strided rows are in flight while the solve runs (the solver's
+    // barriers wait on LDS only)
     T v[M], s[M];
+    T wa[M + 7];
     {
       T w[M + 3];
       load_window<T, M, 1, 2, DIR != 0>(b, t, n, w);
       rhs_chunk<K_DVP, T, M, 1>(w, cfd, v);
     }
+    load_window<T, M, 3, 4, DIR != 0>(a, t, n, wa);
+    __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v, t, ccd, lds);
-    {
-      T w[M + 7];
-      load_window<T, M, 3, 4, DIR != 0>(a, t, n, w);
-      rhs_chunk<K_IVP, T, M, 3>(w, cfi, s);
-    }
+    __builtin_amdgcn_sched_barrier(0);
+    rhs_chunk<K_IVP, T, M, 3>(wa, cfi, s);
+    T wc[M + 7];
+    load_window<T, M, 3, 4, DIR != 0>(c, t, n, wc);
+    __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(s, t, cci, lds);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < M; ++k) v[k] = v[k] + s[k];
     store_chunk<T, M>(o1, t, v);
-    {
-      T w[M + 7];
-      load_window<T, M, 3, 4, DIR != 0>(c, t, n, w);
-      rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
-    }
+    rhs_chunk<K_IVP, T, M, 3>(wc, cfi, v);
     solve_circ<T, M, DIR != 0>(v, t, cci, lds);
     store_chunk<T, M>(o2, t, v);
   } else {
@@ -244,12 +256,12 @@ __device__ __forceinline__ void div_body(const PassArgs<T>& __restrict__ A, int 
       load_window<T, M, 3, 4, DIR != 0>(a, t, n, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
+    T wb[M + 3];
+    load_window<T, M, 1, 2, DIR != 0>(b, t, n, wb);
+    __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v, t, cci, lds);
-    {
-      T w[M + 3];
-      load_window<T, M, 1, 2, DIR != 0>(b, t, n, w);
-      rhs_chunk<K_DVP, T, M, 1>(w, cfd, s);
-    }
+    __builtin_amdgcn_sched_barrier(0);
+    rhs_chunk<K_DVP, T, M, 1>(wb, cfd, s);
     solve_circ<T, M, DIR != 0>(s, t, ccd, lds);
 #pragma unroll
     for (int k = 0; k < M; ++k) v[k] = v[k] + s[k];

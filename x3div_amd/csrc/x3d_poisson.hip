@@ -15,6 +15,7 @@
 // so it is bit-identical to kxyz (src/poisson.f90:673-695) and the null-mode
 // test makes the same decisions.
 #include <cmath>
+#include <cstdlib>
 
 #include "x3d_internal.h"
 
@@ -252,9 +253,157 @@ __global__ __launch_bounds__(NT) void k_poisson_z(C* __restrict__ spec, const do
   }
 }
 
+// The same z stage as a persistent, software-pipelined kernel.  With one
+// work-group per compute unit (a 128 KiB tile) the kernel above loads, then
+// transforms, then stores, and the memory system idles during the
+// transforms.  Here each work-group walks tiles blockIdx.x, +gridDim.x, ...,
+// and the next tile's columns are loaded into registers while the current
+// tile is transformed.  Vector-memory loads complete in issue order, so
+// nothing the transforms wait on may be loaded from global memory after the
+// prefetch: twiddles and the z tables are copied to LDS once, and the x and y
+// factors of a tile are loaded before its successor's prefetch.  Same
+// arithmetic, in the same order, as k_poisson_z.
+template <typename C, int N, int TI, int NT>
+constexpr size_t pzp_lds_bytes()
+{
+  return (size_t)N * TI * sizeof(C) + (size_t)(N / 2 > 0 ? N / 2 : 1) * sizeof(double2) +
+         (size_t)2 * N * sizeof(double);
+}
+
+template <typename C, int N, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_poisson_zp(C* __restrict__ spec, const double2* __restrict__ tw,
+                                                   const double* __restrict__ ktab, int nx, int ny,
+                                                   double inv_ntot)
+{
+#pragma clang fp contract(off)
+  static_assert(NT % TI == 0, "a thread keeps one column");
+  constexpr int NE = N * TI;
+  constexpr int E = (NE + NT - 1) / NT;  // elements per thread
+  constexpr int NTW = N / 2 > 0 ? N / 2 : 1;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf = reinterpret_cast<C*>(smem_raw);
+  double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * NE);
+  double* kz = reinterpret_cast<double*>(twl + NTW);  // zk2[0..N), then tz[0..N)
+  const int nxh = nx / 2 + 1;
+  const int tiles_x = (nxh + TI - 1) / TI;
+  const int ntiles = tiles_x * ny;
+  const long long S = (long long)nxh * ny;  // spectrum stride between z-planes
+  const int tid = threadIdx.x, col = tid % TI;
+  const double* xk2 = ktab;
+  const double* tx = xk2 + nx;
+  const double* yk2 = tx + nx;
+  const double* ty = yk2 + ny;
+  const double* zk2 = ty + ny;
+  const double* tz = zk2 + N;
+  for (int e = tid; e < NTW; e += NT) twl[e] = tw[e];
+  for (int e = tid; e < N; e += NT) {
+    kz[e] = zk2[e];
+    kz[N + e] = tz[e];
+  }
+  // (the tables are read after the first barrier below)
+  C P[E];
+  auto fetch = [&](int tile) {
+    const int j = tile / tiles_x, i0 = (tile - j * tiles_x) * TI;
+    const long long base = (long long)j * nxh + i0;
+    const bool in = i0 + col < nxh;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int e = tid + NT * q;
+      P[q].x = 0;
+      P[q].y = 0;
+      if (e < NE && in) P[q] = spec[(long long)(e / TI) * S + base + col];
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
+    const int j = tile / tiles_x, i0 = (tile - j * tiles_x) * TI, i = i0 + col;
+    const long long base = (long long)j * nxh + i0;
+    const bool in = i < nxh;
+    // this tile's x and y factors, issued before the next prefetch
+    double kxx = 0, txx = 0;
+    if (in) {
+      kxx = xk2[i];
+      txx = tx[i];
+    }
+    const double kyy = yk2[j], tyy = ty[j];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int e = tid + NT * q;
+      if (e < NE) buf[e] = P[q];
+    }
+    __syncthreads();
+    if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+    line_dft<C, N, TI, NT>(buf, twl, false);
+    // (rolled loops below: their state is a few registers, the prefetch is
+    // not, and unrolling them spills)
+#pragma unroll 1
+    for (int q = 0; q < E; ++q) {
+      const int e = tid + NT * q;
+      if (e >= NE || !in) continue;
+      const int k = e / TI;
+      const double tzk = kz[N + k];
+      const double tyz = tyy * tzk, txz = txx * tzk, txy = txx * tyy;
+      const double K = (kxx * (tyz * tyz) + kyy * (txz * txz)) + kz[k] * (txy * txy);
+      C v = buf[e];
+      if (K < 1.0e-16) {
+        v.x = 0;
+        v.y = 0;
+      } else {
+        using R = typename CplxT<C>::R;
+        const double qd = inv_ntot / (-K);
+        v.x = (R)((double)v.x * qd);
+        v.y = (R)((double)v.y * qd);
+      }
+      buf[e] = v;
+    }
+    __syncthreads();
+    line_dft<C, N, TI, NT>(buf, twl, true);
+#pragma unroll 1
+    for (int q = 0; q < E; ++q) {
+      const int e = tid + NT * q;
+      if (e < NE && in) spec[(long long)(e / TI) * S + base + col] = buf[e];
+    }
+    // buf is refilled by the next tile
+    __syncthreads();
+  }
+}
+
+template <typename C, int N, int TI, int NT>
+static int launch_pzp(x3d_ctx* c, C* spec)
+{
+  constexpr size_t smem = pzp_lds_bytes<C, N, TI, NT>();
+  const void* fn = (const void*)k_poisson_zp<C, N, TI, NT>;
+  if (smem > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  // one launch wave of work-groups: compute units x resident work-groups each
+  static int per_cu = 0;
+  if (per_cu <= 0) {
+    int nb = 0;
+    X3D_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, smem));
+    per_cu = nb > 0 ? nb : 1;
+  }
+  int ncu = 0;
+  X3D_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int nxh = c->nx / 2 + 1;
+  const long long ntiles = (long long)((nxh + TI - 1) / TI) * c->ny;
+  const long long cap = (long long)(ncu > 0 ? ncu : 1) * per_cu;
+  const unsigned blocks = (unsigned)(ntiles < cap ? ntiles : cap);
+  const double inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
+  const double2* tw = reinterpret_cast<const double2*>(c->ztw);
+  hipLaunchKernelGGL((k_poisson_zp<C, N, TI, NT>), dim3(blocks), dim3(NT), smem, c->stream, spec, tw, c->ktab,
+                     c->nx, c->ny, inv_ntot);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
 template <typename C, int N, int TI, int NT>
 static int launch_pz(x3d_ctx* c, C* spec)
 {
+  // the pipelined kernel wherever its LDS fits a compute unit (lines up to
+  // 512 points: longer ones leave too few registers for the prefetch)
+  if constexpr (N <= 512 && pzp_lds_bytes<C, N, TI, NT>() <= 160 * 1024) {
+    if (!std::getenv("X3D_PZ_PLAIN")) return launch_pzp<C, N, TI, NT>(c, spec);
+  }
   const int nxh = c->nx / 2 + 1;
   const unsigned blocks = (unsigned)(((nxh + TI - 1) / TI) * c->ny);
   const size_t smem = (size_t)N * TI * sizeof(C);
