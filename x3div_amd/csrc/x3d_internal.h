@@ -86,6 +86,8 @@ struct x3d_ctx {
   double* ktab = nullptr;           // device 1-D tables, see poisson source
   bool zsplit = false;              // 2-D hipFFT plans + fused z stage (nz a power of two)
   void* ztw = nullptr;              // e^{-2 pi i t / nz}, t < nz (double2)
+  bool ysplit = false;              // and x-only hipFFT plans + y stage kernels (ny a power of two <= 512)
+  void* ytw = nullptr;              // e^{-2 pi i t / ny}, t < ny (double2)
   std::vector<double> ax, bx, ay, by, az, bz;
   std::vector<double> xk2, yk2, zk2, exs, eys, ezs;
   // step scratch fields (allocated on first use)

@@ -60,11 +60,13 @@ __global__ void k_spectral(double2* __restrict__ spec, const double* __restrict_
 // With nz a power of two the 3-D transform is split: hipFFT runs the 2-D
 // r2c/c2r over (x, y) of every z-plane, and one kernel does, per z-line of the
 // half spectrum, the forward length-nz DFT, the division above and the
-// inverse DFT, so the spectrum crosses HBM once for the three steps.  A
-// work-group takes 8 neighbouring x-columns of one y (128-byte rows for
-// FP64), holds the 8 lines in LDS and runs Stockham radix-8 passes (a final
-// radix-4 or radix-2 pass when log2 nz is not a multiple of 3) with twiddles
-// from a table of e^{-2 pi i t / nz}.
+// inverse DFT, so the spectrum crosses HBM once for the three steps.  When ny
+// is also a power of two (up to 512), the inverse y stage is a kernel of the
+// same kind and hipFFT runs only the inverse x transform.  A work-group takes TI
+// neighbouring x-columns (16 where they fit in LDS: 256-byte rows for FP64),
+// holds the TI lines in LDS and runs Stockham radix-8 passes (a final
+// radix-4 or radix-2 pass when log2 n is not a multiple of 3) with twiddles
+// from a table of e^{-2 pi i t / n}.
 
 template <typename C>
 struct CplxT;
@@ -138,6 +140,25 @@ __device__ __forceinline__ void small_dft(C (&v)[R], bool inv)
   }
 }
 
+// Twiddles and DFT of butterfly jj (of N / R per line) of a radix-R Stockham
+// pass, Ns = product of the radices already applied.  Its inputs are the
+// points jj + r N / R; its outputs go to (jj / Ns) Ns R + jj % Ns + r Ns.
+template <typename C, int N, int R>
+__device__ __forceinline__ void stockham_bfly(C (&v)[R], const double2* __restrict__ tw, int jj, int Ns, bool inv)
+{
+  const int kin = jj % Ns;
+  const int step = N / (Ns * R);
+  // w^r from one table entry by repeated products (one load per butterfly)
+  const double2 w1 = tw[kin * step];
+  double2 wr = w1;
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    v[r] = cmul(v[r], wr, inv);
+    if (r + 1 < R) wr = double2{wr.x * w1.x - wr.y * w1.y, wr.x * w1.y + wr.y * w1.x};
+  }
+  small_dft<C, R>(v, inv);
+}
+
 // One Stockham pass of radix R over the TI lines held in buf[k][TI]
 // (Ns = product of the radices already applied).
 template <typename C, int N, int R, int TI, int NT>
@@ -162,18 +183,8 @@ __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict_
     const int b = tid + NT * q;
     if (b < NB) {
       const int col = b % TI, jj = b / TI;
-      const int kin = jj % Ns;
-      const int step = N / (Ns * R);
-      // w^r from one table entry by repeated products (one load per butterfly)
-      const double2 w1 = tw[kin * step];
-      double2 wr = w1;
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        v[q][r] = cmul(v[q][r], wr, inv);
-        if (r + 1 < R) wr = double2{wr.x * w1.x - wr.y * w1.y, wr.x * w1.y + wr.y * w1.x};
-      }
-      small_dft<C, R>(v[q], inv);
-      const int d = (jj / Ns) * Ns * R + kin;
+      stockham_bfly<C, N, R>(v[q], tw, jj, Ns, inv);
+      const int d = (jj / Ns) * Ns * R + jj % Ns;
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[(d + r * Ns) * TI + col] = v[q][r];
     }
@@ -253,26 +264,32 @@ __global__ __launch_bounds__(NT) void k_poisson_z(C* __restrict__ spec, const do
   }
 }
 
-// The same z stage as a persistent, software-pipelined kernel.  With one
-// work-group per compute unit (a 128 KiB tile) the kernel above loads, then
-// transforms, then stores, and the memory system idles during the
+// Persistent, software-pipelined line kernels over the half spectrum.  With
+// one work-group per compute unit (a 128 KiB tile) the kernel above loads,
+// then transforms, then stores, and the memory system idles during the
 // transforms.  Here each work-group walks tiles blockIdx.x, +gridDim.x, ...,
 // and the next tile's columns are loaded into registers while the current
 // tile is transformed.  Vector-memory loads complete in issue order, so
 // nothing the transforms wait on may be loaded from global memory after the
 // prefetch: twiddles and the z tables are copied to LDS once, and the x and y
-// factors of a tile are loaded before its successor's prefetch.  Same
-// arithmetic, in the same order, as k_poisson_z.
-template <typename C, int N, int TI, int NT>
-constexpr size_t pzp_lds_bytes()
+// factors of a tile are loaded before its successor's prefetch.
+//   MODE 0 (z-lines): forward DFT, division, inverse DFT; the expressions of
+//          k_poisson_z in the same order (products may fuse into
+//          multiply-adds differently, so results agree to rounding).
+//   MODE 1 / 2 (y-lines): the forward / inverse length-ny DFT alone, for the
+//          y stage of the 3-D transform.  A tile is TI x-columns of one
+//          z-plane, so its rows share the plane's pages.
+// Tile t covers columns i0.. of outer line index o (y for MODE 0, z else).
+template <typename C, int N, int TI, int MODE>
+constexpr size_t spl_lds_bytes()
 {
   return (size_t)N * TI * sizeof(C) + (size_t)(N / 2 > 0 ? N / 2 : 1) * sizeof(double2) +
-         (size_t)2 * N * sizeof(double);
+         (MODE == 0 ? (size_t)2 * N * sizeof(double) : 0);
 }
 
-template <typename C, int N, int TI, int NT>
-__global__ __launch_bounds__(NT) void k_poisson_zp(C* __restrict__ spec, const double2* __restrict__ tw,
-                                                   const double* __restrict__ ktab, int nx, int ny,
+template <typename C, int N, int TI, int NT, int MODE>
+__global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const double2* __restrict__ tw,
+                                                   const double* __restrict__ ktab, int nx, int ny, int nz,
                                                    double inv_ntot)
 {
 #pragma clang fp contract(off)
@@ -280,53 +297,128 @@ __global__ __launch_bounds__(NT) void k_poisson_zp(C* __restrict__ spec, const d
   constexpr int NE = N * TI;
   constexpr int E = (NE + NT - 1) / NT;  // elements per thread
   constexpr int NTW = N / 2 > 0 ? N / 2 : 1;
+  // 512-point lines with one radix-8 butterfly per thread and pass: the
+  // prefetched points of a thread (jj + 64 q of column col) are the inputs of
+  // its first butterfly, and the outputs of its last forward butterfly are the
+  // inputs of its first inverse one and, after the inverse, the points it
+  // stores.  Those passes skip LDS, and the division is applied in registers.
+  constexpr bool REG = N == 512 && TI * (N / 8) == NT;
   extern __shared__ unsigned char smem_raw[];
   C* buf = reinterpret_cast<C*>(smem_raw);
   double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * NE);
-  double* kz = reinterpret_cast<double*>(twl + NTW);  // zk2[0..N), then tz[0..N)
+  double* kz = reinterpret_cast<double*>(twl + NTW);  // MODE 0: zk2[0..N), then tz[0..N)
   const int nxh = nx / 2 + 1;
   const int tiles_x = (nxh + TI - 1) / TI;
-  const int ntiles = tiles_x * ny;
-  const long long S = (long long)nxh * ny;  // spectrum stride between z-planes
+  const int ntiles = tiles_x * (MODE == 0 ? ny : nz);
+  const long long S = MODE == 0 ? (long long)nxh * ny : (long long)nxh;   // between line points
+  const long long OS = MODE == 0 ? (long long)nxh : (long long)nxh * ny;  // between outer indices
   const int tid = threadIdx.x, col = tid % TI;
   const double* xk2 = ktab;
   const double* tx = xk2 + nx;
   const double* yk2 = tx + nx;
   const double* ty = yk2 + ny;
   const double* zk2 = ty + ny;
-  const double* tz = zk2 + N;
+  const double* tz = zk2 + nz;
   for (int e = tid; e < NTW; e += NT) twl[e] = tw[e];
-  for (int e = tid; e < N; e += NT) {
-    kz[e] = zk2[e];
-    kz[N + e] = tz[e];
+  if constexpr (MODE == 0) {
+    for (int e = tid; e < N; e += NT) {
+      kz[e] = zk2[e];
+      kz[N + e] = tz[e];
+    }
   }
   // (the tables are read after the first barrier below)
+  // Element e = tid + NT q of a tile is point tid / TI + (NT / TI) q of
+  // column col: a uniform row address plus this thread's 32-bit byte offset
+  // (the host checks (NT / TI) S sizeof(C) < 4 GiB), so that no 64-bit
+  // address stays live per element.
+  const uint32_t voff = ((uint32_t)(tid / TI) * (uint32_t)S + (uint32_t)col) * (uint32_t)sizeof(C);
+  auto at = [&](long long base, int q) -> C& {
+    C* row = spec + base + (long long)(NT / TI) * q * S;
+    return *reinterpret_cast<C*>(reinterpret_cast<char*>(row) + voff);
+  };
   C P[E];
   auto fetch = [&](int tile) {
-    const int j = tile / tiles_x, i0 = (tile - j * tiles_x) * TI;
-    const long long base = (long long)j * nxh + i0;
+    const int o = tile / tiles_x, i0 = (tile - o * tiles_x) * TI;
+    const long long base = (long long)o * OS + i0;
     const bool in = i0 + col < nxh;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q;
       P[q].x = 0;
       P[q].y = 0;
-      if (e < NE && in) P[q] = spec[(long long)(e / TI) * S + base + col];
+      if (e < NE && in) P[q] = at(base, q);
     }
   };
   int tile = blockIdx.x;
   if (tile < ntiles) fetch(tile);
   for (; tile < ntiles; tile += gridDim.x) {
-    const int j = tile / tiles_x, i0 = (tile - j * tiles_x) * TI, i = i0 + col;
-    const long long base = (long long)j * nxh + i0;
+    const int o = tile / tiles_x, i0 = (tile - o * tiles_x) * TI, i = i0 + col;
+    const long long base = (long long)o * OS + i0;
     const bool in = i < nxh;
     // this tile's x and y factors, issued before the next prefetch
-    double kxx = 0, txx = 0;
-    if (in) {
-      kxx = xk2[i];
-      txx = tx[i];
+    double kxx = 0, txx = 0, kyy = 0, tyy = 0;
+    if constexpr (MODE == 0) {
+      if (in) {
+        kxx = xk2[i];
+        txx = tx[i];
+      }
+      kyy = yk2[o];
+      tyy = ty[o];
     }
-    const double kyy = yk2[j], tyy = ty[j];
+    if constexpr (REG) {
+      // radix-8 passes with Ns = 1, 8, 64; thread = butterfly jj of column col
+      const int jj = tid / TI;
+      C v[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = P[r];  // points jj + 64 r
+      constexpr bool inv0 = MODE == 2;
+      stockham_bfly<C, N, 8>(v, twl, jj, 1, inv0);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) buf[(8 * jj + r) * TI + col] = v[r];
+      // the prefetch once v is in LDS, so that the two are not live together
+      if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+      __syncthreads();
+      stockham_pass<C, N, 8, TI, NT>(buf, twl, 8, inv0);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = buf[(jj + 64 * r) * TI + col];
+      stockham_bfly<C, N, 8>(v, twl, jj, 64, inv0);  // outputs: points jj + 64 r
+      if constexpr (MODE == 0) {
+        __syncthreads();  // every thread has read buf before it is rewritten
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          if (!in) continue;
+          const int k = jj + 64 * r;
+          const double tzk = kz[N + k];
+          const double tyz = tyy * tzk, txz = txx * tzk, txy = txx * tyy;
+          const double K = (kxx * (tyz * tyz) + kyy * (txz * txz)) + kz[k] * (txy * txy);
+          if (K < 1.0e-16) {
+            v[r].x = 0;
+            v[r].y = 0;
+          } else {
+            using R = typename CplxT<C>::R;
+            const double qd = inv_ntot / (-K);
+            v[r].x = (R)((double)v[r].x * qd);
+            v[r].y = (R)((double)v[r].y * qd);
+          }
+        }
+        // the inverse's first butterfly takes the same points jj + 64 r
+        stockham_bfly<C, N, 8>(v, twl, jj, 1, true);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[(8 * jj + r) * TI + col] = v[r];
+        __syncthreads();
+        stockham_pass<C, N, 8, TI, NT>(buf, twl, 8, true);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = buf[(jj + 64 * r) * TI + col];
+        stockham_bfly<C, N, 8>(v, twl, jj, 64, true);
+      }
+      if (in) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) at(base, r) = v[r];  // point jj + 64 r
+      }
+      // buf is refilled by the next tile
+      __syncthreads();
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q;
@@ -334,46 +426,77 @@ __global__ __launch_bounds__(NT) void k_poisson_zp(C* __restrict__ spec, const d
     }
     __syncthreads();
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-    line_dft<C, N, TI, NT>(buf, twl, false);
-    // (rolled loops below: their state is a few registers, the prefetch is
-    // not, and unrolling them spills)
+    if constexpr (MODE != 2) line_dft<C, N, TI, NT>(buf, twl, false);
+    if constexpr (MODE == 0) {
+      // (rolled loops below: their state is a few registers, the prefetch
+      // is not, and unrolling them spills)
 #pragma unroll 1
-    for (int q = 0; q < E; ++q) {
-      const int e = tid + NT * q;
-      if (e >= NE || !in) continue;
-      const int k = e / TI;
-      const double tzk = kz[N + k];
-      const double tyz = tyy * tzk, txz = txx * tzk, txy = txx * tyy;
-      const double K = (kxx * (tyz * tyz) + kyy * (txz * txz)) + kz[k] * (txy * txy);
-      C v = buf[e];
-      if (K < 1.0e-16) {
-        v.x = 0;
-        v.y = 0;
-      } else {
-        using R = typename CplxT<C>::R;
-        const double qd = inv_ntot / (-K);
-        v.x = (R)((double)v.x * qd);
-        v.y = (R)((double)v.y * qd);
+      for (int q = 0; q < E; ++q) {
+        const int e = tid + NT * q;
+        if (e >= NE || !in) continue;
+        const int k = e / TI;
+        const double tzk = kz[N + k];
+        const double tyz = tyy * tzk, txz = txx * tzk, txy = txx * tyy;
+        const double K = (kxx * (tyz * tyz) + kyy * (txz * txz)) + kz[k] * (txy * txy);
+        C v = buf[e];
+        if (K < 1.0e-16) {
+          v.x = 0;
+          v.y = 0;
+        } else {
+          using R = typename CplxT<C>::R;
+          const double qd = inv_ntot / (-K);
+          v.x = (R)((double)v.x * qd);
+          v.y = (R)((double)v.y * qd);
+        }
+        buf[e] = v;
       }
-      buf[e] = v;
+      __syncthreads();
     }
-    __syncthreads();
-    line_dft<C, N, TI, NT>(buf, twl, true);
+    if constexpr (MODE != 1) line_dft<C, N, TI, NT>(buf, twl, true);
 #pragma unroll 1
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q;
-      if (e < NE && in) spec[(long long)(e / TI) * S + base + col] = buf[e];
+      if (e < NE && in) at(base, q) = buf[e];
     }
     // buf is refilled by the next tile
     __syncthreads();
   }
 }
 
-template <typename C, int N, int TI, int NT>
-static int launch_pzp(x3d_ctx* c, C* spec)
+// Tile (measured at 512^3): 16 columns where 16 lines fit in 128 KiB of LDS,
+// else 8; 1024 threads for tiles of 64 KiB and more (the work-group is then
+// the compute unit's only one or one of two, and its own waves hide the
+// latency), 256 otherwise.
+template <typename C, int N>
+struct LineTile {
+  static constexpr int TI = ((size_t)N * 16 * sizeof(C) <= 131072) ? 16 : 8;
+  static constexpr int NT = (size_t)N * TI * sizeof(C) >= 65536 ? 1024 : 256;
+};
+
+// Point stride of the line kernels (elements) and whether their 32-bit
+// per-thread offsets, (NT / TI) strides, fit.
+template <typename C, int N, int MODE>
+static bool spl_off32_ok(const x3d_ctx* c)
 {
-  constexpr size_t smem = pzp_lds_bytes<C, N, TI, NT>();
-  const void* fn = (const void*)k_poisson_zp<C, N, TI, NT>;
+  const long long S = (long long)(c->nx / 2 + 1) * (MODE == 0 ? c->ny : 1);
+  return (long long)(LineTile<C, N>::NT / LineTile<C, N>::TI) * S * (long long)sizeof(C) < (1LL << 32);
+}
+
+// The pipelined kernels cover lines of up to 512 points (longer ones leave
+// too few registers for the prefetch) whose LDS fits a compute unit.
+template <typename C, int N, int MODE>
+constexpr bool spl_ok()
+{
+  return N >= 8 && N <= 512 && spl_lds_bytes<C, N, LineTile<C, N>::TI, MODE>() <= 160 * 1024;
+}
+
+template <typename C, int N, int MODE>
+static int launch_spl(x3d_ctx* c, C* spec, const void* twdev)
+{
+  constexpr int TI = LineTile<C, N>::TI, NT = LineTile<C, N>::NT;
+  constexpr size_t smem = spl_lds_bytes<C, N, TI, MODE>();
+  if (!spl_off32_ok<C, N, MODE>(c)) return X3D_ERR_UNSUPPORTED;
+  const void* fn = (const void*)k_spec_lines<C, N, TI, NT, MODE>;
   if (smem > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   // one launch wave of work-groups: compute units x resident work-groups each
   static int per_cu = 0;
@@ -385,24 +508,22 @@ static int launch_pzp(x3d_ctx* c, C* spec)
   int ncu = 0;
   X3D_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
   const int nxh = c->nx / 2 + 1;
-  const long long ntiles = (long long)((nxh + TI - 1) / TI) * c->ny;
+  const long long ntiles = (long long)((nxh + TI - 1) / TI) * (MODE == 0 ? c->ny : c->nz);
   const long long cap = (long long)(ncu > 0 ? ncu : 1) * per_cu;
   const unsigned blocks = (unsigned)(ntiles < cap ? ntiles : cap);
   const double inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
-  const double2* tw = reinterpret_cast<const double2*>(c->ztw);
-  hipLaunchKernelGGL((k_poisson_zp<C, N, TI, NT>), dim3(blocks), dim3(NT), smem, c->stream, spec, tw, c->ktab,
-                     c->nx, c->ny, inv_ntot);
+  hipLaunchKernelGGL((k_spec_lines<C, N, TI, NT, MODE>), dim3(blocks), dim3(NT), smem, c->stream, spec,
+                     reinterpret_cast<const double2*>(twdev), c->ktab, c->nx, c->ny, c->nz, inv_ntot);
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
 
-template <typename C, int N, int TI, int NT>
-static int launch_pz(x3d_ctx* c, C* spec)
+template <typename C, int N>
+static int launch_pz_n(x3d_ctx* c, C* spec)
 {
-  // the pipelined kernel wherever its LDS fits a compute unit (lines up to
-  // 512 points: longer ones leave too few registers for the prefetch)
-  if constexpr (N <= 512 && pzp_lds_bytes<C, N, TI, NT>() <= 160 * 1024) {
-    if (!std::getenv("X3D_PZ_PLAIN")) return launch_pzp<C, N, TI, NT>(c, spec);
+  constexpr int TI = LineTile<C, N>::TI, NT = LineTile<C, N>::NT;
+  if constexpr (spl_ok<C, N, 0>()) {
+    if (!std::getenv("X3D_PZ_PLAIN") && spl_off32_ok<C, N, 0>(c)) return launch_spl<C, N, 0>(c, spec, c->ztw);
   }
   const int nxh = c->nx / 2 + 1;
   const unsigned blocks = (unsigned)(((nxh + TI - 1) / TI) * c->ny);
@@ -418,16 +539,21 @@ static int launch_pz(x3d_ctx* c, C* spec)
   return X3D_OK;
 }
 
-// Tile (measured at 512^3): 16 columns where 16 lines fit in 128 KiB of LDS,
-// else 8; 1024 threads for tiles of 64 KiB and more (the work-group is then
-// the compute unit's only one or one of two, and its own waves hide the
-// latency), 256 otherwise.
-template <typename C, int N>
-static int launch_pz_n(x3d_ctx* c, C* spec)
+// y stage of the 3-D transform (MODE 1 forward, 2 inverse), for the y
+// lengths ysplit_ok admits
+template <typename C, int MODE>
+static int launch_poisson_y(x3d_ctx* c, C* spec)
 {
-  constexpr int TI = ((size_t)N * 16 * sizeof(C) <= 131072) ? 16 : 8;
-  constexpr int NT = (size_t)N * TI * sizeof(C) >= 65536 ? 1024 : 256;
-  return launch_pz<C, N, TI, NT>(c, spec);
+  switch (c->ny) {
+    case 8: return launch_spl<C, 8, MODE>(c, spec, c->ytw);
+    case 16: return launch_spl<C, 16, MODE>(c, spec, c->ytw);
+    case 32: return launch_spl<C, 32, MODE>(c, spec, c->ytw);
+    case 64: return launch_spl<C, 64, MODE>(c, spec, c->ytw);
+    case 128: return launch_spl<C, 128, MODE>(c, spec, c->ytw);
+    case 256: return launch_spl<C, 256, MODE>(c, spec, c->ytw);
+    case 512: return launch_spl<C, 512, MODE>(c, spec, c->ytw);
+    default: return X3D_ERR_UNSUPPORTED;
+  }
 }
 
 template <typename C>
@@ -449,6 +575,48 @@ static int launch_poisson_z(x3d_ctx* c, C* spec)
 #define X3D_TRY_RC(expr) do { const int _r = (expr); if (_r) return _r; } while (0)
 
 static bool zsplit_ok(int nz) { return nz >= 8 && nz <= 1024 && (nz & (nz - 1)) == 0; }
+static bool ysplit_ok(int ny) { return ny >= 8 && ny <= 512 && (ny & (ny - 1)) == 0; }
+
+// Table of e^{-2 pi i t / n}, t < n, on the device.
+static int twiddles(int n, void** dev)
+{
+  std::vector<double2> tw(n);
+  for (int t = 0; t < n; ++t) {
+    const double a = -2.0 * PI * (double)t / (double)n;
+    tw[t] = double2{std::cos(a), std::sin(a)};
+  }
+  X3D_CHECK_HIP(hipMalloc(dev, tw.size() * sizeof(double2)));
+  X3D_CHECK_HIP(hipMemcpy(*dev, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return X3D_OK;
+}
+
+// hipFFT plans of the stages it runs.  Forward: the 2-D (x, y) transform of
+// every z-plane when the z stage is this file's kernel (hipFFT fuses the x
+// r2c into one pass there, while its 1-D batched r2c takes two), else the
+// whole 3-D transform.  Inverse: the x c2r of every row when the y stage is
+// this file's kernel too, else the 2-D resp. 3-D transform.
+static bool make_plans(const x3d_ctx* c, hipfftHandle* r2c, hipfftHandle* c2r, hipfftType tr, hipfftType tc)
+{
+  const int nx = c->nx, ny = c->ny, nz = c->nz, nxh = nx / 2 + 1;
+  int n2[2] = {ny, nx}, re2[2] = {ny, nx}, ce2[2] = {ny, nxh};
+  const int rd = nx * ny, cd = nxh * ny;
+  if (c->zsplit) {
+    if (hipfftPlanMany(r2c, 2, n2, re2, 1, rd, ce2, 1, cd, tr, nz) != HIPFFT_SUCCESS) return false;
+  } else if (hipfftPlan3d(r2c, nz, ny, nx, tr) != HIPFFT_SUCCESS) {
+    return false;
+  }
+  bool ok;
+  if (c->ysplit) {
+    int n1[1] = {nx}, re1[1] = {nx}, ce1[1] = {nxh};
+    ok = hipfftPlanMany(c2r, 1, n1, ce1, 1, nxh, re1, 1, nx, tc, ny * nz) == HIPFFT_SUCCESS;
+  } else if (c->zsplit) {
+    ok = hipfftPlanMany(c2r, 2, n2, ce2, 1, cd, re2, 1, rd, tc, nz) == HIPFFT_SUCCESS;
+  } else {
+    ok = hipfftPlan3d(c2r, nz, ny, nx, tc) == HIPFFT_SUCCESS;
+  }
+  if (!ok) hipfftDestroy(*r2c);
+  return ok;
+}
 
 // 1-D tables of abxyz (src/poisson.f90:436-476) and waves (:522-616):
 // modified wavenumber squared of the staggered derivative and the
@@ -499,25 +667,11 @@ int poisson_setup(x3d_ctx* c)
   const size_t ns = (size_t)(c->nx / 2 + 1) * c->ny * c->nz;
   X3D_CHECK_HIP(hipMalloc(&c->spec, ns * sizeof(double2)));
   c->zsplit = zsplit_ok(c->nz);
-  if (c->zsplit) {
-    // 2-D transforms of every z-plane; the z stage is the fused kernel
-    std::vector<double2> tw(c->nz);
-    for (int t = 0; t < c->nz; ++t) {
-      const double a = -2.0 * PI * (double)t / (double)c->nz;
-      tw[t] = double2{std::cos(a), std::sin(a)};
-    }
-    X3D_CHECK_HIP(hipMalloc(&c->ztw, tw.size() * sizeof(double2)));
-    X3D_CHECK_HIP(hipMemcpy(c->ztw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
-    int n2[2] = {c->ny, c->nx}, re[2] = {c->ny, c->nx}, ce[2] = {c->ny, c->nx / 2 + 1};
-    const int rd = c->nx * c->ny, cd = (c->nx / 2 + 1) * c->ny;
-    if (hipfftPlanMany(&c->fft_r2c, 2, n2, re, 1, rd, ce, 1, cd, HIPFFT_D2Z, c->nz) != HIPFFT_SUCCESS)
-      return X3D_ERR_FFT;
-    if (hipfftPlanMany(&c->fft_c2r, 2, n2, ce, 1, cd, re, 1, rd, HIPFFT_Z2D, c->nz) != HIPFFT_SUCCESS)
-      return X3D_ERR_FFT;
-  } else {
-    if (hipfftPlan3d(&c->fft_r2c, c->nz, c->ny, c->nx, HIPFFT_D2Z) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
-    if (hipfftPlan3d(&c->fft_c2r, c->nz, c->ny, c->nx, HIPFFT_Z2D) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
-  }
+  // (the y kernel's 32-bit offsets span 64 rows of the half spectrum)
+  c->ysplit = c->zsplit && ysplit_ok(c->ny) && (long long)(c->nx / 2 + 1) * 64 * 16 < (1LL << 32);
+  if (c->zsplit) X3D_TRY_RC(twiddles(c->nz, &c->ztw));
+  if (c->ysplit) X3D_TRY_RC(twiddles(c->ny, &c->ytw));
+  if (!make_plans(c, &c->fft_r2c, &c->fft_c2r, HIPFFT_D2Z, HIPFFT_Z2D)) return X3D_ERR_FFT;
   c->fft_ready = true;
   if (hipfftSetStream(c->fft_r2c, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (hipfftSetStream(c->fft_c2r, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
@@ -539,9 +693,11 @@ void poisson_release(x3d_ctx* c)
   if (c->spec) (void)hipFree(c->spec);
   if (c->ktab) (void)hipFree(c->ktab);
   if (c->ztw) (void)hipFree(c->ztw);
+  if (c->ytw) (void)hipFree(c->ytw);
   c->spec = nullptr;
   c->ktab = nullptr;
   c->ztw = nullptr;
+  c->ytw = nullptr;
 }
 
 // FP32 spectrum, FP64 division: the modified wavenumber and the quotient are
@@ -583,19 +739,7 @@ int poisson_prepare_f32(x3d_ctx* c)
 {
   if (!c->fft_ready) return X3D_ERR_FFT;
   if (!c->fft32_ready) {
-    int n2[2] = {c->ny, c->nx}, re[2] = {c->ny, c->nx}, ce[2] = {c->ny, c->nx / 2 + 1};
-    const int rd = c->nx * c->ny, cd = (c->nx / 2 + 1) * c->ny;
-    const bool ok1 = c->zsplit ? hipfftPlanMany(&c->fft_r2c32, 2, n2, re, 1, rd, ce, 1, cd, HIPFFT_R2C,
-                                                c->nz) == HIPFFT_SUCCESS
-                               : hipfftPlan3d(&c->fft_r2c32, c->nz, c->ny, c->nx, HIPFFT_R2C) == HIPFFT_SUCCESS;
-    if (!ok1) return X3D_ERR_FFT;
-    const bool ok2 = c->zsplit ? hipfftPlanMany(&c->fft_c2r32, 2, n2, ce, 1, cd, re, 1, rd, HIPFFT_C2R,
-                                                c->nz) == HIPFFT_SUCCESS
-                               : hipfftPlan3d(&c->fft_c2r32, c->nz, c->ny, c->nx, HIPFFT_C2R) == HIPFFT_SUCCESS;
-    if (!ok2) {
-      hipfftDestroy(c->fft_r2c32);
-      return X3D_ERR_FFT;
-    }
+    if (!make_plans(c, &c->fft_r2c32, &c->fft_c2r32, HIPFFT_R2C, HIPFFT_C2R)) return X3D_ERR_FFT;
     c->fft32_ready = true;
   }
   return X3D_OK;
@@ -611,8 +755,8 @@ int poisson_solve_f32(x3d_ctx* c, float* rhs)
   float2* spec = reinterpret_cast<float2*>(c->spec);
   if (hipfftExecR2C(c->fft_r2c32, rhs, (hipfftComplex*)spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (c->zsplit) {
-    const int rc2 = launch_poisson_z<float2>(c, spec);
-    if (rc2) return rc2;
+    X3D_TRY_RC(launch_poisson_z<float2>(c, spec));
+    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<float2, 2>(c, spec)));
     if (hipfftExecC2R(c->fft_c2r32, (hipfftComplex*)spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
     return X3D_OK;
   }
@@ -633,6 +777,7 @@ int poisson_solve(x3d_ctx* c, double* rhs)
   if (hipfftExecD2Z(c->fft_r2c, rhs, (hipfftDoubleComplex*)c->spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (c->zsplit) {
     X3D_TRY_RC(launch_poisson_z<double2>(c, c->spec));
+    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<double2, 2>(c, c->spec)));
     if (hipfftExecZ2D(c->fft_c2r, (hipfftDoubleComplex*)c->spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
     return X3D_OK;
   }
