@@ -276,7 +276,7 @@ __global__ __launch_bounds__(NT) void k_poisson_z(C* __restrict__ spec, const do
 //   MODE 0 (z-lines): forward DFT, division, inverse DFT; the expressions of
 //          k_poisson_z in the same order (products may fuse into
 //          multiply-adds differently, so results agree to rounding).
-//   MODE 1 / 2 (y-lines): the forward / inverse length-ny DFT alone, for the
+//   MODE 2 (y-lines): the inverse length-ny DFT alone, for the
 //          y stage of the 3-D transform.  A tile is TI x-columns of one
 //          z-plane, so its rows share the plane's pages.
 // Tile t covers columns i0.. of outer line index o (y for MODE 0, z else).
@@ -294,6 +294,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
 {
 #pragma clang fp contract(off)
   static_assert(NT % TI == 0, "a thread keeps one column");
+  static_assert(MODE == 0 || MODE == 2, "z stage or inverse y stage");
   constexpr int NE = N * TI;
   constexpr int E = (NE + NT - 1) / NT;  // elements per thread
   constexpr int NTW = N / 2 > 0 ? N / 2 : 1;
@@ -302,7 +303,9 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
   // its first butterfly, and the outputs of its last forward butterfly are the
   // inputs of its first inverse one and, after the inverse, the points it
   // stores.  Those passes skip LDS, and the division is applied in registers.
-  constexpr bool REG = N == 512 && TI * (N / 8) == NT;
+  // (FP64 y-lines excepted: with only the inverse transform after the
+  // prefetch, which here starts one pass later, too little work hides it)
+  constexpr bool REG = N == 512 && TI * (N / 8) == NT && !(MODE == 2 && sizeof(C) == 16);
   extern __shared__ unsigned char smem_raw[];
   C* buf = reinterpret_cast<C*>(smem_raw);
   double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * NE);
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
       C v[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = P[r];  // points jj + 64 r
-      constexpr bool inv0 = MODE == 2;
+      constexpr bool inv0 = MODE == 2;  // z: forward first; y: inverse only
       stockham_bfly<C, N, 8>(v, twl, jj, 1, inv0);
 #pragma unroll
       for (int r = 0; r < 8; ++r) buf[(8 * jj + r) * TI + col] = v[r];
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
     }
     __syncthreads();
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-    if constexpr (MODE != 2) line_dft<C, N, TI, NT>(buf, twl, false);
+    if constexpr (MODE == 0) line_dft<C, N, TI, NT>(buf, twl, false);
     if constexpr (MODE == 0) {
       // (rolled loops below: their state is a few registers, the prefetch
       // is not, and unrolling them spills)
@@ -452,7 +455,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
       }
       __syncthreads();
     }
-    if constexpr (MODE != 1) line_dft<C, N, TI, NT>(buf, twl, true);
+    line_dft<C, N, TI, NT>(buf, twl, true);
 #pragma unroll 1
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q;
@@ -539,19 +542,19 @@ static int launch_pz_n(x3d_ctx* c, C* spec)
   return X3D_OK;
 }
 
-// y stage of the 3-D transform (MODE 1 forward, 2 inverse), for the y
+// inverse y stage of the 3-D transform, for the y
 // lengths ysplit_ok admits
-template <typename C, int MODE>
+template <typename C>
 static int launch_poisson_y(x3d_ctx* c, C* spec)
 {
   switch (c->ny) {
-    case 8: return launch_spl<C, 8, MODE>(c, spec, c->ytw);
-    case 16: return launch_spl<C, 16, MODE>(c, spec, c->ytw);
-    case 32: return launch_spl<C, 32, MODE>(c, spec, c->ytw);
-    case 64: return launch_spl<C, 64, MODE>(c, spec, c->ytw);
-    case 128: return launch_spl<C, 128, MODE>(c, spec, c->ytw);
-    case 256: return launch_spl<C, 256, MODE>(c, spec, c->ytw);
-    case 512: return launch_spl<C, 512, MODE>(c, spec, c->ytw);
+    case 8: return launch_spl<C, 8, 2>(c, spec, c->ytw);
+    case 16: return launch_spl<C, 16, 2>(c, spec, c->ytw);
+    case 32: return launch_spl<C, 32, 2>(c, spec, c->ytw);
+    case 64: return launch_spl<C, 64, 2>(c, spec, c->ytw);
+    case 128: return launch_spl<C, 128, 2>(c, spec, c->ytw);
+    case 256: return launch_spl<C, 256, 2>(c, spec, c->ytw);
+    case 512: return launch_spl<C, 512, 2>(c, spec, c->ytw);
     default: return X3D_ERR_UNSUPPORTED;
   }
 }
@@ -756,7 +759,7 @@ int poisson_solve_f32(x3d_ctx* c, float* rhs)
   if (hipfftExecR2C(c->fft_r2c32, rhs, (hipfftComplex*)spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (c->zsplit) {
     X3D_TRY_RC(launch_poisson_z<float2>(c, spec));
-    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<float2, 2>(c, spec)));
+    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<float2>(c, spec)));
     if (hipfftExecC2R(c->fft_c2r32, (hipfftComplex*)spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
     return X3D_OK;
   }
@@ -777,7 +780,7 @@ int poisson_solve(x3d_ctx* c, double* rhs)
   if (hipfftExecD2Z(c->fft_r2c, rhs, (hipfftDoubleComplex*)c->spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (c->zsplit) {
     X3D_TRY_RC(launch_poisson_z<double2>(c, c->spec));
-    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<double2, 2>(c, c->spec)));
+    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<double2>(c, c->spec)));
     if (hipfftExecZ2D(c->fft_c2r, (hipfftDoubleComplex*)c->spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
     return X3D_OK;
   }
