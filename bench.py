@@ -155,6 +155,31 @@ def time_form(ctx, f, step_fn, timed_fn, args):
     return ms_step, gpu_ms_step, passes
 
 
+def measure_operators(ctx, src, dst, reps=10):
+    """Each of the 18 FP64 drop-in operators (x3d_apply: one RHS + periodic
+    line solve pass, 8 B read + 8 B written per point) timed alone: average
+    of `reps` launches between HIP events on the library's stream, against
+    the 8 TB/s HBM peak."""
+    import torch
+    from x3div_amd import OPS  # the reference's 18 operator instances, in order
+    n3 = ctx.nx * ctx.ny * ctx.nz
+    out = {}
+    for name in OPS:
+        for _ in range(2):
+            ctx.apply(name, dst, src)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(ctx.stream)
+        for _ in range(reps):
+            ctx.apply(name, dst, src)
+        ev1.record(ctx.stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        gbs = 16 * n3 / (ms / 1e3) / 1e9
+        out[name] = {"ms": round(ms, 4), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def roofline_of(passes, n, gpu_ms_step, suffix, npts=None):
     """The slowest line pass (the Poisson pass is several hipFFT kernels and is
     reported beside it) against the HBM peak, plus the whole step's fraction."""
@@ -352,6 +377,8 @@ def main():
     forms = {"fp64": {"value": round(value, 2), "ms_per_step": round(ms_step, 4), "dtype": "f64",
                       "step_frac": roofline["step_frac"]}}
     log(f"fp64 {n}^3: {ms_step:.3f} ms/step")
+    operators = measure_operators(ctx, f[0], f[3])
+    log("fp64 single operators done")
     if "fp32" in extra:
         forms["fp32"] = measure_fp32(ctx, u0, args, ms_step)
         log(f"fp32 {n}^3: {forms['fp32']['ms_per_step']:.3f} ms/step")
@@ -388,6 +415,7 @@ def main():
             "parallelism": "1 GPU (prow=pcol=1)",
         },
         "roofline": roofline,
+        "operators_fp64": operators,
         "forms": forms,
         "cpu_baseline": cpu,
     }
