@@ -4,7 +4,8 @@ Inputs are seeded and identical on both sides (built on the host, uploaded).
 Floating-point tolerance: 1e-12 relative in the max norm, the FP64 bar for the
 derivatives and the timestep; the transposes are compared bit for bit.
 Grids are chosen to exercise every line tiling: chunk lengths M = 1..5,
-chunks per line C = 8..64, lines per wavefront 1..16, odd and even x chunks
+chunks per line C = 8..64 (including the 64-chunk fallback of y/z plans),
+lines per wavefront 1..16, odd and even x chunks
 (even ones move two points per access), and different domain lengths so
 that an axis mix-up cannot cancel.
 """
@@ -21,6 +22,9 @@ GRIDS = [
     (40, 96, 160, 2 * np.pi, 1.0, 0.5),
     (16, 8, 24, 1.0, 1.0, 1.0),
     (128, 24, 16, 1.0, 1.0, 2.0),
+    # 320-point y-lines: no split into at most 32 chunks of a compiled length,
+    # so the y plans take 64 chunks of 5 points
+    (12, 320, 40, 1.0, 3.0, 1.5),
 ]
 
 
