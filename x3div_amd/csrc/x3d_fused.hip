@@ -223,30 +223,27 @@ __device__ __forceinline__ void div_body(const PassArgs<T>& __restrict__ A, int 
     solve_circ<T, M, false>(v, t, cci, lds);
     x_store<T, M>(o3, xb, v);
   } else if constexpr (DIR == 1) {
-    // each input window is loaded before the preceding line solve, so that
-    // its strided rows are in flight while the solve runs (the solver's
-    // barriers wait on LDS only)
     T v[M], s[M];
-    T wa[M + 7];
     {
       T w[M + 3];
       load_window<T, M, 1, 2, DIR != 0>(b, t, n, w);
       rhs_chunk<K_DVP, T, M, 1>(w, cfd, v);
     }
-    load_window<T, M, 3, 4, DIR != 0>(a, t, n, wa);
-    __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v, t, ccd, lds);
-    __builtin_amdgcn_sched_barrier(0);
-    rhs_chunk<K_IVP, T, M, 3>(wa, cfi, s);
-    T wc[M + 7];
-    load_window<T, M, 3, 4, DIR != 0>(c, t, n, wc);
-    __builtin_amdgcn_sched_barrier(0);
+    {
+      T w[M + 7];
+      load_window<T, M, 3, 4, DIR != 0>(a, t, n, w);
+      rhs_chunk<K_IVP, T, M, 3>(w, cfi, s);
+    }
     solve_circ<T, M, DIR != 0>(s, t, cci, lds);
-    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < M; ++k) v[k] = v[k] + s[k];
     store_chunk<T, M>(o1, t, v);
-    rhs_chunk<K_IVP, T, M, 3>(wc, cfi, v);
+    {
+      T w[M + 7];
+      load_window<T, M, 3, 4, DIR != 0>(c, t, n, w);
+      rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
+    }
     solve_circ<T, M, DIR != 0>(v, t, cci, lds);
     store_chunk<T, M>(o2, t, v);
   } else {
@@ -352,10 +349,12 @@ __device__ __forceinline__ int grid_of(const int* __restrict__ block0, int ngrid
 
 template <typename T, int M, int DIR>
 __global__ __launch_bounds__(512) void k_transeq(TranseqArgs<T> A) { transeq_body<T, M, DIR>(A, blockIdx.x); }
+// FP64 y passes: at most 128 VGPRs, so that two work-groups share a compute
+// unit (measured faster at 512^3 despite a few spills)
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) void k_div(PassArgs<T> A) { div_body<T, M, DIR>(A, blockIdx.x); }
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 && sizeof(T) == 8 ? 4 : 1))) void k_div(PassArgs<T> A) { div_body<T, M, DIR>(A, blockIdx.x); }
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) void k_grad(PassArgs<T> A) { grad_body<T, M, DIR>(A, blockIdx.x); }
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 && sizeof(T) == 8 ? 4 : 1))) void k_grad(PassArgs<T> A) { grad_body<T, M, DIR>(A, blockIdx.x); }
 
 template <typename T, int M, int DIR>
 __global__ __launch_bounds__(512) void k_transeq_batch(const TranseqArgs<T>* __restrict__ list,
