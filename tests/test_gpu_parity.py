@@ -30,7 +30,7 @@ GRIDS = [
     # the z-line Poisson kernel with register first/last passes, the y one
     (512, 8, 8, 2 * np.pi, 1.0, 1.0),
     (8, 512, 8, 1.0, 2 * np.pi, 1.0),
-    (8, 8, 512, 1.0, 1.0, 2 * np.pi),
+    (8, 8, 512, 1.0, 2.0, 2 * np.pi),
 ]
 
 
@@ -182,16 +182,12 @@ def test_step_parity(grid, gdt):
         assert relerr(host(a), b) < RTOL
     # The pressure solves on div u*, a 1e-3-scale difference of O(1) terms, so
     # rounding is amplified by that cancellation: L2-relative at the FP64 bar,
-    # max-norm one decade looser.  On grids where the cancellation is deeper
-    # (8x8x512: a pressure 1e3 times smaller) the bar is 10x the oracle's own
-    # change under a 1-ulp perturbation of the input fields.  The velocity it
-    # corrects is checked above at 1e-12 in the max norm.
-    rng = np.random.default_rng(7)
-    up = [a * (1 + 2.0 ** -52 * rng.standard_normal(a.shape)) for a in u0]
-    ppu = orc.step(*up, gdt=gdt)[2]
-    c2, cm = relerr2(ppu, rpp), relerr(ppu, rpp)
-    assert relerr2(host(pp), rpp) < max(RTOL, 10 * c2), c2
-    assert relerr(host(pp), rpp) < max(10 * RTOL, 10 * cm), cm
+    # max-norm one decade looser.  The velocity it corrects is checked above
+    # at 1e-12 in the max norm.  (With equal x and y lengths the Taylor-Green
+    # part is discretely solenoidal and the pressure, from the noise alone, is
+    # 1e3 times smaller and that much worse conditioned; the grids avoid it.)
+    assert relerr2(host(pp), rpp) < RTOL
+    assert relerr(host(pp), rpp) < 10 * RTOL
 
 
 def test_caller_tables_take_the_table_sweeps(grid):
