@@ -89,6 +89,7 @@ struct x3d_ctx {
   bool ysplit = false;              // and x-only hipFFT plans + y stage kernels (ny a power of two <= 512)
   void* ytw = nullptr;              // e^{-2 pi i t / ny}, t < ny (double2)
   int ncu = 0;                      // compute units of the context's device
+  int spl_wg[2][2] = {{0, 0}, {0, 0}};  // resident work-groups/CU of the Poisson line kernels [FP64/FP32][z/y]
   std::vector<double> ax, bx, ay, by, az, bz;
   std::vector<double> xk2, yk2, zk2, exs, eys, ezs;
   // step scratch fields (allocated on first use)

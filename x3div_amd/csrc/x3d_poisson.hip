@@ -493,32 +493,36 @@ constexpr bool spl_ok()
   return N >= 8 && N <= 512 && spl_lds_bytes<C, N, LineTile<C, N>::TI, MODE>() <= 160 * 1024;
 }
 
-// Resident work-groups per compute unit of a line kernel (and its LDS limit
-// raised once), found at context set-up so that a captured step makes no
-// such call; -1 on a HIP error.
+// Slot of a line kernel's resident work-groups per compute unit in the
+// context: [FP64 / FP32][z stage / inverse y stage].
+template <typename C, int MODE>
+static int& spl_slot(x3d_ctx* c)
+{
+  return c->spl_wg[sizeof(C) == 16 ? 0 : 1][MODE == 0 ? 0 : 1];
+}
+
+// At context set-up, on the context's device: raise the line kernel's LDS
+// limit and record its resident work-groups per compute unit, so that a
+// launch (possibly under graph capture) makes no such call.
 template <typename C, int N, int MODE>
-static int spl_per_cu()
+static int spl_prepare_n(x3d_ctx* c)
 {
   constexpr int TI = LineTile<C, N>::TI, NT = LineTile<C, N>::NT;
   constexpr size_t smem = spl_lds_bytes<C, N, TI, MODE>();
-  static int per_cu = 0;
-  if (per_cu <= 0) {
-    const void* fn = (const void*)k_spec_lines<C, N, TI, NT, MODE>;
-    if (smem > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem) != hipSuccess)
-      return -1;
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, smem) != hipSuccess) return -1;
-    per_cu = nb > 0 ? nb : 1;
-  }
-  return per_cu;
+  const void* fn = (const void*)k_spec_lines<C, N, TI, NT, MODE>;
+  if (smem > 65536) X3D_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  int nb = 0;
+  X3D_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, smem));
+  spl_slot<C, MODE>(c) = nb > 0 ? nb : 1;
+  return X3D_OK;
 }
 
 template <typename C, int MODE>
-static int spl_prepare(int n)
+static int spl_prepare(x3d_ctx* c, int n)
 {
   switch (n) {
 #define X3D_SPL_N(NN) \
-    case NN: if constexpr (spl_ok<C, NN, MODE>()) return spl_per_cu<C, NN, MODE>() < 0 ? X3D_ERR_HIP : X3D_OK; break;
+    case NN: if constexpr (spl_ok<C, NN, MODE>()) return spl_prepare_n<C, NN, MODE>(c); break;
     X3D_SPL_N(8) X3D_SPL_N(16) X3D_SPL_N(32) X3D_SPL_N(64) X3D_SPL_N(128) X3D_SPL_N(256) X3D_SPL_N(512)
 #undef X3D_SPL_N
     default: break;
@@ -532,8 +536,8 @@ static int launch_spl(x3d_ctx* c, C* spec, const void* twdev)
   constexpr int TI = LineTile<C, N>::TI, NT = LineTile<C, N>::NT;
   constexpr size_t smem = spl_lds_bytes<C, N, TI, MODE>();
   if (!spl_off32_ok<C, N, MODE>(c)) return X3D_ERR_UNSUPPORTED;
-  const int per_cu = spl_per_cu<C, N, MODE>();
-  if (per_cu < 0) return X3D_ERR_HIP;
+  const int per_cu = spl_slot<C, MODE>(c);
+  if (per_cu <= 0) return X3D_ERR_HIP;  // set-up did not prepare this kernel
   // one launch wave of work-groups: compute units x resident work-groups each
   const int nxh = c->nx / 2 + 1;
   const long long ntiles = (long long)((nxh + TI - 1) / TI) * (MODE == 0 ? c->ny : c->nz);
@@ -701,8 +705,8 @@ int poisson_setup(x3d_ctx* c)
   if (c->ysplit) X3D_TRY_RC(twiddles(c->ny, &c->ytw));
   if (!make_plans(c, &c->fft_r2c, &c->fft_c2r, HIPFFT_D2Z, HIPFFT_Z2D)) return X3D_ERR_FFT;
   X3D_CHECK_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device));
-  if (c->zsplit) X3D_TRY_RC((spl_prepare<double2, 0>(c->nz)));
-  if (c->ysplit) X3D_TRY_RC((spl_prepare<double2, 2>(c->ny)));
+  if (c->zsplit) X3D_TRY_RC((spl_prepare<double2, 0>(c, c->nz)));
+  if (c->ysplit) X3D_TRY_RC((spl_prepare<double2, 2>(c, c->ny)));
   c->fft_ready = true;
   if (hipfftSetStream(c->fft_r2c, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (hipfftSetStream(c->fft_c2r, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
@@ -771,8 +775,8 @@ int poisson_prepare_f32(x3d_ctx* c)
   if (!c->fft_ready) return X3D_ERR_FFT;
   if (!c->fft32_ready) {
     if (!make_plans(c, &c->fft_r2c32, &c->fft_c2r32, HIPFFT_R2C, HIPFFT_C2R)) return X3D_ERR_FFT;
-    if (c->zsplit) X3D_TRY_RC((spl_prepare<float2, 0>(c->nz)));
-    if (c->ysplit) X3D_TRY_RC((spl_prepare<float2, 2>(c->ny)));
+    if (c->zsplit) X3D_TRY_RC((spl_prepare<float2, 0>(c, c->nz)));
+    if (c->ysplit) X3D_TRY_RC((spl_prepare<float2, 2>(c, c->ny)));
     c->fft32_ready = true;
   }
   return X3D_OK;
