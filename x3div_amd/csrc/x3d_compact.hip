@@ -219,11 +219,23 @@ static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStre
     else                                                                                               \
       hipLaunchKernelGGL((k_line<T, KIND, MM, CIRC, false>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
     break;
+  // FP32 y/z lines also take the FP32 plans' 24- and 32-point chunks
+#define X3D_CASE_F32_YZ(MM)                                                                            \
+  case MM:                                                                                             \
+    if constexpr (sizeof(T) == 4) {                                                                    \
+      if (!tl.line_fast) return hipErrorInvalidValue;                                                  \
+      hipLaunchKernelGGL((k_line<T, KIND, MM, CIRC, true>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
+      break;                                                                                           \
+    } else {                                                                                           \
+      return hipErrorInvalidValue;                                                                     \
+    }
   switch (M) {
     X3D_CASE(1) X3D_CASE(2) X3D_CASE(3) X3D_CASE(4) X3D_CASE(5) X3D_CASE(6)
     X3D_CASE(8) X3D_CASE(12) X3D_CASE(16)
+    X3D_CASE_F32_YZ(24) X3D_CASE_F32_YZ(32)
     default: return hipErrorInvalidValue;
   }
+#undef X3D_CASE_F32_YZ
 #undef X3D_CASE
   return hipGetLastError();
 }
@@ -284,7 +296,9 @@ int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double*
 int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS || !in || !out || kind == K_NONE) return X3D_ERR_ARG;
-  const LinePlan& P = c->plan[dir];
+  // y/z: the FP32 plans' tiling (chunks up to 32 points, 128-byte rows of 32
+  // lines where the line allows) when there is one, as in the FP32 step
+  const LinePlan& P = dir > 0 && c->fplan32[dir].M > 0 ? c->fplan32[dir] : c->plan[dir];
   const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
   if (c->op[op].n != n || P.n != n) return X3D_ERR_SIZE;
   if (!c->op[op].circulant) return X3D_ERR_UNSUPPORTED;
@@ -292,7 +306,7 @@ int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, floa
   Tiling tl;
   dim3 grid, block;
   size_t smem;
-  launch_dims(c, dir, &geo, &tl, &grid, &block, &smem, sizeof(float));
+  launch_dims_plan(c, dir, P, &geo, &tl, &grid, &block, &smem, sizeof(float));
   const Coef<double> k = stencil_coef(c->sc[dir], kind);
   const double ig = 1.0 / circ_gamma(c->op[op].alfa);
   const Coef<float> cf{(float)(k.a * ig), (float)(k.b * ig), (float)(k.c * ig), (float)(k.d * ig)};
