@@ -329,7 +329,10 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
       kz[N + e] = tz[e];
     }
   }
-  // (the tables are read after the first barrier below)
+  // The tables are written by other wavefronts' threads: every wavefront
+  // waits for them here, since the 512-point path's first butterfly reads
+  // the twiddles before the tile loop's first barrier.
+  __syncthreads();
   // Element e = tid + NT q of a tile is point tid / TI + (NT / TI) q of
   // column col: a uniform row address plus this thread's 32-bit byte offset
   // (the host checks (NT / TI) S sizeof(C) < 4 GiB), so that no 64-bit
