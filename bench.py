@@ -10,8 +10,9 @@ Prints ONE JSON line on stdout.  Besides the throughput it carries
   roofline:     the dominant kernel's algorithmic bytes per launch divided by
                 its average launch time (HIP events on the library's stream),
                 against the 8 TB/s HBM peak;
-  cpu_baseline: the CPU oracle (single-threaded C restatement of the
-                reference) timed on a bounded sample of the same workload.
+  cpu_baseline: the CPU oracle (C restatement of the reference, built with
+                -O3 and OpenMP over lines) timed on the host's threads and
+                on one thread, on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -36,7 +37,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512, help="grid points per direction")
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--cpu-n", type=int, default=256, help="grid of the CPU-baseline sample step")
+    ap.add_argument("--cpu-n", type=int, default=256, help="grid of the all-threads CPU-baseline steps")
+    ap.add_argument("--cpu-n1", type=int, default=128, help="grid of the one-thread CPU-baseline steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forms", default="fp32,batch,large",
                     help="comma list of extra forms to measure beside plain FP64: fp32 (precision "
@@ -93,22 +95,61 @@ def traffic_from_profile(pass_name: str, n: int):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(n: int):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_steps(n: int, threads: int, steps: int):
+    """Per-step seconds of the oracle's speed build (-O3, OpenMP over lines) on
+    `threads` host threads: steps + 1 Euler steps on n^3 seeded TG + 1e-3
+    fields (L = 2pi), the first dropped (page faults, first touch)."""
     import numpy as np
 
-    from oracle import Oracle
-    o = Oracle(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi)
+    import oracle
+    oracle.set_threads(threads, fast=True)
+    o = oracle.Oracle(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, fast=True)
     u = o.fields(1, 1234, 1e-3)
-    t0 = time.perf_counter()
-    o.step(*u, gdt=0.0)
-    dt = time.perf_counter() - t0
+    du = [o.zeros() for _ in range(3)]
+    pp = o.zeros()
+    g = [o.zeros() for _ in range(3)]
+    times = []
+    for _ in range(steps + 1):
+        t0 = time.perf_counter()
+        o.step_inplace(u, du, pp, g, gdt=0.0)
+        times.append(time.perf_counter() - t0)
+    return times[1:]
+
+
+def cpu_baseline(n: int, n1: int, steps: int = 5):
+    """The C oracle's speed build timed on this host: on every thread the
+    process may use (OMP_NUM_THREADS) at n^3, and on one thread at n1^3;
+    median of `steps` steps after a dropped first one.  The reference's own
+    per-step timing is tend - tstart around one step (src/xcompact3d.f90:78,98)."""
+    import statistics
+
+    import oracle
+    cores = oracle.max_threads(fast=True)
+    t_all = cpu_steps(n, cores, steps)
+    t_one = cpu_steps(n1, 1, steps)
+    med_all = statistics.median(t_all)
+    med_one = statistics.median(t_one)
     return {
-        "value": round(n ** 3 / dt / 1e6, 4),
+        "value": round(n ** 3 / med_all / 1e6, 3),
         "unit": "Mpts/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": f"one full Euler step of the C oracle at {n}^3 (seeded TG + 1e-3 noise, L=2pi), "
-                  f"single thread, {dt:.1f} s",
+        "cpu_model": cpu_model(),
+        "sample": f"median of {steps} Euler steps (after one dropped) of the C oracle's -O3 OpenMP build "
+                  f"at {n}^3, seeded TG + 1e-3 noise, L=2pi, {cores} threads: {med_all:.2f} s/step",
+        "single_thread": {"value": round(n1 ** 3 / med_one / 1e6, 3), "grid": n1,
+                          "s_per_step": round(med_one, 3)},
     }
 
 
@@ -392,7 +433,7 @@ def main():
         forms.update(measure_large(args))
         log("large grids done")
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_n)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_n, args.cpu_n1)
     out = {
         "metric": "Mpts/s per Euler timestep at 512\u00b3 (plain FP64 vs FP32/split-BF16) + HBM GB/s",
         "value": round(value, 2),

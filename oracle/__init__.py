@@ -16,6 +16,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborcx3d.so")
+# the same source built for speed (-O3, FMA contraction): the CPU baseline only
+FAST_LIB_PATH = os.path.join(HERE, "liborcx3d_fast.so")
 
 # operator ids (src/x3d_operator_1d.f90:33-47); identical to the HIP library's
 OPS = ["derx", "derxp", "dery", "deryp", "derz", "derzp",
@@ -27,21 +29,21 @@ KIND = {"d": 0, "dvp": 1, "dpv": 2, "ivp": 3, "ipv": 4}
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (gcc, no FP contraction)."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "liborcx3d.so"],
+    """Compile the checker (gcc, no FP contraction) and the fast baseline build."""
+    if force or not (os.path.exists(LIB_PATH) and os.path.exists(FAST_LIB_PATH)):
+        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "all"],
                        check=True, capture_output=True)
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
+def lib(fast: bool = False):
+    """The checker library, or with fast=True the speed build (baseline timing only)."""
+    if fast not in _libs:
         build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(FAST_LIB_PATH if fast else LIB_PATH)
         dp = C.POINTER(C.c_double)
         vp = C.c_void_p
         L.orc_new.restype = vp
@@ -76,8 +78,10 @@ def lib():
         L.orc_test_err.restype = C.c_double
         L.orc_test_err.argtypes = [vp, dp, C.c_int]
         L.orc_div_stats.argtypes = [vp, dp, dp, dp]
-        _lib = L
-    return _lib
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_max_threads.restype = C.c_int
+        _libs[fast] = L
+    return _libs[fast]
 
 
 def _p(a: np.ndarray):
@@ -89,16 +93,17 @@ class Oracle:
     """One periodic grid: coefficients, operators, Poisson and the step."""
 
     def __init__(self, nx: int, ny: int, nz: int, xlx: float = 1.0, yly: float = 1.0,
-                 zlz: float = 1.0):
+                 zlz: float = 1.0, fast: bool = False):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.L = (xlx, yly, zlz)
-        self._c = lib().orc_new(nx, ny, nz, xlx, yly, zlz)
+        self._L = lib(fast)
+        self._c = self._L.orc_new(nx, ny, nz, xlx, yly, zlz)
         if not self._c:
             raise ValueError(f"oracle: unsupported grid {nx}x{ny}x{nz}")
 
     def __del__(self):
         if getattr(self, "_c", None):
-            lib().orc_delete(self._c)
+            self._L.orc_delete(self._c)
             self._c = None
 
     @property
@@ -113,27 +118,27 @@ class Oracle:
         n = {"x": self.nx, "y": self.ny, "z": self.nz}[_dir_letter(name)]
         f, s, w, p = (np.zeros(n) for _ in range(4))
         alfa = C.c_double(); nn = C.c_int(); npaire = C.c_int()
-        rc = lib().orc_get_op(self._c, OP[name], _p(f), _p(s), _p(w), _p(p),
+        rc = self._L.orc_get_op(self._c, OP[name], _p(f), _p(s), _p(w), _p(p),
                               C.byref(alfa), C.byref(nn), C.byref(npaire))
         assert rc == 0 and nn.value == n
         return f, s, w, p, alfa.value, npaire.value
 
     def consts(self):
         out = np.zeros(33)
-        lib().orc_get_consts(self._c, _p(out))
+        self._L.orc_get_consts(self._c, _p(out))
         names = ["alfai", "afi", "bfi", "alcai", "aci", "bci", "ailcai", "aici", "bici", "cici", "dici"]
         return [dict(zip(names, out[11 * d:11 * d + 11])) for d in range(3)]
 
     def kxyz(self):
         """(Re, Im) of kxyz on the (nx, ny, nz/2+1) spectral grid."""
         re = np.zeros((self.nz // 2 + 1, self.ny, self.nx)); im = np.zeros_like(re)
-        lib().orc_get_kxyz(self._c, _p(re), _p(im))
+        self._L.orc_get_kxyz(self._c, _p(re), _p(im))
         return re, im
 
     def ab(self):
         """ax, bx, ay, by, az, bz of abxyz."""
         out = np.zeros(2 * (self.nx + self.ny + self.nz))
-        lib().orc_get_ab(self._c, _p(out))
+        self._L.orc_get_ab(self._c, _p(out))
         sizes = [self.nx, self.nx, self.ny, self.ny, self.nz, self.nz]
         parts = np.split(out, np.cumsum(sizes)[:-1])
         return dict(zip(["ax", "bx", "ay", "by", "az", "bz"], parts))
@@ -145,7 +150,7 @@ class Oracle:
             kind = _default_kind(name)
         u = np.ascontiguousarray(u, dtype=np.float64)
         t = np.empty_like(u)
-        rc = lib().orc_apply(self._c, _p(t), _p(u), OP[name], d, KIND[kind], self.nx, self.ny, self.nz)
+        rc = self._L.orc_apply(self._c, _p(t), _p(u), OP[name], d, KIND[kind], self.nx, self.ny, self.nz)
         if rc:
             raise ValueError(f"orc_apply rc={rc}")
         return t
@@ -153,50 +158,50 @@ class Oracle:
     def thomas(self, name: str, tt: np.ndarray) -> np.ndarray:
         d = "xyz".index(_dir_letter(name))
         tt = np.array(tt, dtype=np.float64, order="C", copy=True)
-        rc = lib().orc_thomas(self._c, _p(tt), OP[name], d, self.nx, self.ny, self.nz)
+        rc = self._L.orc_thomas(self._c, _p(tt), OP[name], d, self.nx, self.ny, self.nz)
         assert rc == 0
         return tt
 
     def r2c(self, rhs):
         cw = np.zeros((self.nz // 2 + 1, self.ny, self.nx), dtype=np.complex128)
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
-        lib().orc_fft_r2c_z(self._c, _p(rhs), cw.ctypes.data_as(C.POINTER(C.c_double)))
+        self._L.orc_fft_r2c_z(self._c, _p(rhs), cw.ctypes.data_as(C.POINTER(C.c_double)))
         return cw
 
     def c2r(self, cw):
         cw = np.array(cw, dtype=np.complex128, order="C", copy=True)
         out = self.zeros()
-        lib().orc_fft_c2r_z(self._c, cw.ctypes.data_as(C.POINTER(C.c_double)), _p(out))
+        self._L.orc_fft_c2r_z(self._c, cw.ctypes.data_as(C.POINTER(C.c_double)), _p(out))
         return out
 
     def spectral(self, cw):
         cw = np.array(cw, dtype=np.complex128, order="C", copy=True)
-        lib().orc_poisson_spectral(self._c, cw.ctypes.data_as(C.POINTER(C.c_double)))
+        self._L.orc_poisson_spectral(self._c, cw.ctypes.data_as(C.POINTER(C.c_double)))
         return cw
 
     def poisson(self, rhs):
         out = np.array(rhs, dtype=np.float64, order="C", copy=True)
-        lib().orc_poisson(self._c, _p(out))
+        self._L.orc_poisson(self._c, _p(out))
         return out
 
     def momentum_rhs(self, ux, uy, uz):
         du = [self.zeros() for _ in range(3)]
-        lib().orc_momentum_rhs(self._c, *(_p(a) for a in du), *(_p(np.ascontiguousarray(a)) for a in (ux, uy, uz)))
+        self._L.orc_momentum_rhs(self._c, *(_p(a) for a in du), *(_p(np.ascontiguousarray(a)) for a in (ux, uy, uz)))
         return du
 
     def divergence(self, ux, uy, uz):
         pp = self.zeros()
-        lib().orc_divergence(self._c, _p(pp), *(_p(np.ascontiguousarray(a)) for a in (ux, uy, uz)))
+        self._L.orc_divergence(self._c, _p(pp), *(_p(np.ascontiguousarray(a)) for a in (ux, uy, uz)))
         return pp
 
     def gradp(self, pp):
         g = [self.zeros() for _ in range(3)]
-        lib().orc_gradp(self._c, *(_p(a) for a in g), _p(np.ascontiguousarray(pp)))
+        self._L.orc_gradp(self._c, *(_p(a) for a in g), _p(np.ascontiguousarray(pp)))
         return g
 
     def solve_poisson(self, ux, uy, uz):
         pp = self.zeros(); g = [self.zeros() for _ in range(3)]
-        lib().orc_solve_poisson(self._c, _p(pp), *(_p(a) for a in g),
+        self._L.orc_solve_poisson(self._c, _p(pp), *(_p(a) for a in g),
                                 *(_p(np.ascontiguousarray(a)) for a in (ux, uy, uz)))
         return pp, g
 
@@ -205,21 +210,26 @@ class Oracle:
         u = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (ux, uy, uz)]
         du = [self.zeros() for _ in range(3)]
         pp = self.zeros(); g = [self.zeros() for _ in range(3)]
-        lib().orc_step(self._c, gdt, *(_p(a) for a in u), *(_p(a) for a in du), _p(pp),
+        self._L.orc_step(self._c, gdt, *(_p(a) for a in u), *(_p(a) for a in du), _p(pp),
                        *(_p(a) for a in g))
         return u, du, pp, g
 
+    def step_inplace(self, u, du, pp, g, gdt: float = 0.0) -> None:
+        """One Euler step on caller-owned arrays (u updated; du, pp, g overwritten)."""
+        self._L.orc_step(self._c, gdt, *(_p(a) for a in u), *(_p(a) for a in du), _p(pp),
+                         *(_p(a) for a in g))
+
     def fields(self, kind: int = 1, seed: int = 1234, amp: float = 1e-3):
         u = [self.zeros() for _ in range(3)]
-        lib().orc_init_fields(self._c, kind, seed, amp, *(_p(a) for a in u))
+        self._L.orc_init_fields(self._c, kind, seed, amp, *(_p(a) for a in u))
         return u
 
     def test_err(self, d, which: int) -> float:
-        return lib().orc_test_err(self._c, _p(np.ascontiguousarray(d)), which)
+        return self._L.orc_test_err(self._c, _p(np.ascontiguousarray(d)), which)
 
     def div_stats(self, pp):
         mx = C.c_double(); mean = C.c_double()
-        lib().orc_div_stats(self._c, _p(np.ascontiguousarray(pp)), C.byref(mx), C.byref(mean))
+        self._L.orc_div_stats(self._c, _p(np.ascontiguousarray(pp)), C.byref(mx), C.byref(mean))
         return mx.value, mean.value
 
 
@@ -240,6 +250,15 @@ def prepare(b, c, f):
     b, c, f = (np.ascontiguousarray(a, dtype=np.float64) for a in (b, c, f))
     lib().orc_prepare(_p(b), _p(c), _p(f), _p(s), _p(w), n)
     return s, w
+
+
+def set_threads(n: int, fast: bool = False) -> None:
+    """OpenMP threads of the oracle's line loops (results do not depend on it)."""
+    lib(fast).orc_set_threads(int(n))
+
+
+def max_threads(fast: bool = False) -> int:
+    return lib(fast).orc_max_threads()
 
 
 def uniform(seed: int, comp: int, idx: int) -> float:

@@ -12,6 +12,7 @@
 
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -180,6 +181,7 @@ static void poisson_tables(orc_ctx *c)
     c->ezs[k] = (double)nz * w / c->zlz;
   }
   const int nzh = nz / 2 + 1;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nzh; ++k) {
     double rlezs = c->ezs[k] * c->dz;
     double ztt = 2.0 * (c->bici[2] * cos(rlezs * 1.5) + c->cici[2] * cos(rlezs * 2.5) + c->dici[2] * cos(rlezs * 3.5));
@@ -308,6 +310,7 @@ void orc_get_ab(const orc_ctx *c, double *out)
 void orc_xthomas_12(double *tt, const double *ff, const double *fs, const double *fw,
                     int nx, int ny, int nz)
 {
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j) {
       double *t = tt + IDX(0, j, k, nx, ny);
@@ -321,6 +324,7 @@ void orc_xthomas_0(double *tt, const orc_op *op, int nx, int ny, int nz)
 {
   orc_xthomas_12(tt, op->f, op->s, op->w, nx, ny, nz);
   const double *p = op->periodic; const double a = op->alfa;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j) {
       double *t = tt + IDX(0, j, k, nx, ny);
@@ -329,23 +333,32 @@ void orc_xthomas_0(double *tt, const orc_op *op, int nx, int ny, int nz)
     }
 }
 
+/* The y and z sweeps run the lines of a plane side by side (i innermost);
+ * each line sees the same operations in the same order as the reference's
+ * line-by-line sweeps (src/thomas.f90:114-136, 166-188). */
 void orc_ythomas_12(double *tt, const double *ff, const double *fs, const double *fw,
                     int nx, int ny, int nz)
 {
-  for (int k = 0; k < nz; ++k)
-    for (int i = 0; i < nx; ++i) {
-      for (int j = 1; j < ny; ++j)
-        tt[IDX(i, j, k, nx, ny)] = tt[IDX(i, j, k, nx, ny)] - tt[IDX(i, j - 1, k, nx, ny)] * fs[j];
-      tt[IDX(i, ny - 1, k, nx, ny)] = tt[IDX(i, ny - 1, k, nx, ny)] * fw[ny - 1];
-      for (int j = ny - 2; j >= 0; --j)
-        tt[IDX(i, j, k, nx, ny)] = (tt[IDX(i, j, k, nx, ny)] - ff[j] * tt[IDX(i, j + 1, k, nx, ny)]) * fw[j];
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; ++k) {
+    for (int j = 1; j < ny; ++j) {
+      double *t = tt + IDX(0, j, k, nx, ny); const double *tm = t - nx;
+      for (int i = 0; i < nx; ++i) t[i] = t[i] - tm[i] * fs[j];
     }
+    double *tl = tt + IDX(0, ny - 1, k, nx, ny);
+    for (int i = 0; i < nx; ++i) tl[i] = tl[i] * fw[ny - 1];
+    for (int j = ny - 2; j >= 0; --j) {
+      double *t = tt + IDX(0, j, k, nx, ny); const double *tp = t + nx;
+      for (int i = 0; i < nx; ++i) t[i] = (t[i] - ff[j] * tp[i]) * fw[j];
+    }
+  }
 }
 
 void orc_ythomas_0(double *tt, const orc_op *op, int nx, int ny, int nz)
 {
   orc_ythomas_12(tt, op->f, op->s, op->w, nx, ny, nz);
   const double *p = op->periodic; const double a = op->alfa;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; ++k)
     for (int i = 0; i < nx; ++i) {
       double ss = (tt[IDX(i, 0, k, nx, ny)] - a * tt[IDX(i, ny - 1, k, nx, ny)]) / (1.0 + p[0] - a * p[ny - 1]);
@@ -356,25 +369,38 @@ void orc_ythomas_0(double *tt, const orc_op *op, int nx, int ny, int nz)
 void orc_zthomas_12(double *tt, const double *ff, const double *fs, const double *fw,
                     int nx, int ny, int nz)
 {
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      for (int k = 1; k < nz; ++k)
-        tt[IDX(i, j, k, nx, ny)] = tt[IDX(i, j, k, nx, ny)] - tt[IDX(i, j, k - 1, nx, ny)] * fs[k];
-      tt[IDX(i, j, nz - 1, nx, ny)] = tt[IDX(i, j, nz - 1, nx, ny)] * fw[nz - 1];
-      for (int k = nz - 2; k >= 0; --k)
-        tt[IDX(i, j, k, nx, ny)] = (tt[IDX(i, j, k, nx, ny)] - ff[k] * tt[IDX(i, j, k + 1, nx, ny)]) * fw[k];
+  const size_t pl = (size_t)nx * ny;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; ++j) {
+    double *t0 = tt + IDX(0, j, 0, nx, ny);
+    for (int k = 1; k < nz; ++k) {
+      double *t = t0 + k * pl; const double *tm = t - pl;
+      for (int i = 0; i < nx; ++i) t[i] = t[i] - tm[i] * fs[k];
     }
+    double *tl = t0 + (nz - 1) * pl;
+    for (int i = 0; i < nx; ++i) tl[i] = tl[i] * fw[nz - 1];
+    for (int k = nz - 2; k >= 0; --k) {
+      double *t = t0 + k * pl; const double *tp = t + pl;
+      for (int i = 0; i < nx; ++i) t[i] = (t[i] - ff[k] * tp[i]) * fw[k];
+    }
+  }
 }
 
 void orc_zthomas_0(double *tt, const orc_op *op, int nx, int ny, int nz)
 {
   orc_zthomas_12(tt, op->f, op->s, op->w, nx, ny, nz);
   const double *p = op->periodic; const double a = op->alfa;
-  for (int j = 0; j < ny; ++j)
+  const size_t pl = (size_t)nx * ny;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < ny; ++j) {
+    double *t0 = tt + IDX(0, j, 0, nx, ny);
+    const double *tl = t0 + (nz - 1) * pl;
+    const double den = 1.0 + p[0] - a * p[nz - 1];
     for (int i = 0; i < nx; ++i) {
-      double ss = (tt[IDX(i, j, 0, nx, ny)] - a * tt[IDX(i, j, nz - 1, nx, ny)]) / (1.0 + p[0] - a * p[nz - 1]);
-      for (int k = 0; k < nz; ++k) tt[IDX(i, j, k, nx, ny)] = tt[IDX(i, j, k, nx, ny)] - ss * p[k];
+      double ss = (t0[i] - a * tl[i]) / den;
+      for (int k = 0; k < nz; ++k) t0[i + k * pl] = t0[i + k * pl] - ss * p[k];
     }
+  }
 }
 
 int orc_thomas(const orc_ctx *c, double *tt, int op, int dir, int nx, int ny, int nz)
@@ -418,6 +444,27 @@ static double rhs_point(const double *u, long stride, int i, int n, int kind,
 #undef U
 }
 
+/* The stencil of rhs_point over a run of len points: U[o + 4] points to the
+ * operand at offset o of the run's first point (o = -4..4), consecutive
+ * points are consecutive in memory.  Same expressions, same operand order. */
+static void rhs_run(double *t, const double *const *U, int len, int kind,
+                    double a, double b, double cc, double d)
+{
+  switch (kind) {
+  case 0: for (int i = 0; i < len; ++i) t[i] = a * (U[5][i] - U[3][i]) + b * (U[6][i] - U[2][i]); break;
+  case 1: for (int i = 0; i < len; ++i) t[i] = a * (U[5][i] - U[4][i]) + b * (U[6][i] - U[3][i]); break;
+  case 2: for (int i = 0; i < len; ++i) t[i] = a * (U[4][i] - U[3][i]) + b * (U[5][i] - U[2][i]); break;
+  case 3:
+    for (int i = 0; i < len; ++i)
+      t[i] = a * (U[5][i] + U[4][i]) + b * (U[6][i] + U[3][i]) + cc * (U[7][i] + U[2][i]) + d * (U[8][i] + U[1][i]);
+    break;
+  default:
+    for (int i = 0; i < len; ++i)
+      t[i] = a * (U[4][i] + U[3][i]) + b * (U[5][i] + U[2][i]) + cc * (U[6][i] + U[1][i]) + d * (U[7][i] + U[0][i]);
+    break;
+  }
+}
+
 int orc_apply(const orc_ctx *c, double *t, const double *u, int op, int dir,
               int kind, int nx, int ny, int nz)
 {
@@ -429,14 +476,26 @@ int orc_apply(const orc_ctx *c, double *t, const double *u, int op, int dir,
   int n = dir == 0 ? nx : dir == 1 ? ny : nz;
   if (c->ops[op].n != n) return -2;
   long stride = dir == 0 ? 1 : dir == 1 ? nx : (long)nx * ny;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; ++k)
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) {
-        int pos = dir == 0 ? i : dir == 1 ? j : k;
-        size_t q = IDX(i, j, k, nx, ny);
-        const double *line = u + q - (size_t)pos * stride;
-        t[q] = rhs_point(line, stride, pos, n, kind, a, b, cc, d);
+    for (int j = 0; j < ny; ++j) {
+      size_t q = IDX(0, j, k, nx, ny);
+      const double *U[9];
+      if (dir == 0) {
+        /* interior points of the x-line as one run, the wrapped ends point by point */
+        const double *line = u + q;
+        for (int o = -4; o <= 4; ++o) U[o + 4] = line + 4 + o;
+        if (nx > 8) rhs_run(t + q + 4, U, nx - 8, kind, a, b, cc, d);
+        for (int i = 0; i < nx; ++i)
+          if (i < 4 || i >= nx - 4 || nx <= 8) t[q + i] = rhs_point(line, 1, i, n, kind, a, b, cc, d);
+      } else {
+        /* a whole x-row at once: each offset is one wrapped neighbour row */
+        int pos = dir == 1 ? j : k;
+        const double *row0 = u + q - (size_t)pos * stride;
+        for (int o = -4; o <= 4; ++o) U[o + 4] = row0 + (long)wrap(pos + o, n) * stride;
+        rhs_run(t + q, U, nx, kind, a, b, cc, d);
       }
+    }
   return orc_thomas(c, t, op, dir, nx, ny, nz);
 }
 
@@ -477,12 +536,36 @@ static void fft_rec(const cplx *in, cplx *out, int n, long stride, const cplx *t
   if (t != tmp) free(t);
 }
 
+/* Power-of-two n: iterative radix-2 decimation in time on a bit-reversed
+ * copy, the same butterflies as fft_rec's p = 2 level without its recursion
+ * and index arithmetic. */
+static void fft_pow2(const cplx *in, cplx *out, int n, const cplx *tw)
+{
+  int lg = 0;
+  while ((1 << lg) < n) ++lg;
+  for (int j = 0; j < n; ++j) {
+    unsigned r = 0, v = (unsigned)j;
+    for (int b = 0; b < lg; ++b) { r = (r << 1) | (v & 1u); v >>= 1; }
+    out[r] = in[j];
+  }
+  for (int h = 1; h < n; h <<= 1) {
+    const int step = n / (2 * h);
+    for (int s0 = 0; s0 < n; s0 += 2 * h)
+      for (int k = 0; k < h; ++k) {
+        cplx e = out[s0 + k], o = out[s0 + k + h] * tw[k * step];
+        out[s0 + k] = e + o;
+        out[s0 + k + h] = e - o;
+      }
+  }
+}
+
 /* y[k] = sum_j x[j*stride] exp(sign 2 pi i j k / n), result written back with stride */
 static void fft_line(cplx *x, int n, long stride, int sign, cplx *work, const cplx *tw)
 {
   (void)sign;
   for (int j = 0; j < n; ++j) work[n + j] = x[(long)j * stride];
-  fft_rec(work + n, work, n, 1, tw, n);
+  if ((n & (n - 1)) == 0) fft_pow2(work + n, work, n, tw);
+  else fft_rec(work + n, work, n, 1, tw, n);
   for (int j = 0; j < n; ++j) x[(long)j * stride] = work[j];
 }
 
@@ -497,26 +580,34 @@ static cplx *twiddles(int n, int sign)
 }
 
 /* r2c, physical data in z-pencil: z first (halved), then y, then x.
- * cw1 layout: (nx, ny, nz/2+1) complex, x fastest. */
+ * cw1 layout: (nx, ny, nz/2+1) complex, x fastest.  Lines are independent,
+ * so they are spread over threads, each with its own work buffers. */
 void orc_fft_r2c_z(const orc_ctx *c, const double *rhs, double *cw1d)
 {
   const int nx = c->nx, ny = c->ny, nz = c->nz, nzh = nz / 2 + 1;
   cplx *cw1 = (cplx *)cw1d;
   int nmax = nx > ny ? nx : ny; nmax = nmax > nz ? nmax : nz;
-  cplx *work = malloc(sizeof(cplx) * 2 * nmax);
-  cplx *line = malloc(sizeof(cplx) * nz);
   cplx *twz = twiddles(nz, -1), *twy = twiddles(ny, -1), *twx = twiddles(nx, -1);
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      for (int k = 0; k < nz; ++k) line[k] = rhs[IDX(i, j, k, nx, ny)];
-      fft_line(line, nz, 1, -1, work, twz);
-      for (int k = 0; k < nzh; ++k) cw1[IDX(i, j, k, nx, ny)] = line[k];
-    }
-  for (int k = 0; k < nzh; ++k)
-    for (int i = 0; i < nx; ++i) fft_line(cw1 + IDX(i, 0, k, nx, ny), ny, nx, -1, work, twy);
-  for (int k = 0; k < nzh; ++k)
-    for (int j = 0; j < ny; ++j) fft_line(cw1 + IDX(0, j, k, nx, ny), nx, 1, -1, work, twx);
-  free(work); free(line); free(twz); free(twy); free(twx);
+#pragma omp parallel
+  {
+    cplx *work = malloc(sizeof(cplx) * 2 * nmax);
+    cplx *line = malloc(sizeof(cplx) * nz);
+#pragma omp for schedule(static)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        for (int k = 0; k < nz; ++k) line[k] = rhs[IDX(i, j, k, nx, ny)];
+        fft_line(line, nz, 1, -1, work, twz);
+        for (int k = 0; k < nzh; ++k) cw1[IDX(i, j, k, nx, ny)] = line[k];
+      }
+#pragma omp for schedule(static)
+    for (int k = 0; k < nzh; ++k)
+      for (int i = 0; i < nx; ++i) fft_line(cw1 + IDX(i, 0, k, nx, ny), ny, nx, -1, work, twy);
+#pragma omp for schedule(static)
+    for (int k = 0; k < nzh; ++k)
+      for (int j = 0; j < ny; ++j) fft_line(cw1 + IDX(0, j, k, nx, ny), nx, 1, -1, work, twx);
+    free(work); free(line);
+  }
+  free(twz); free(twy); free(twx);
 }
 
 /* c2r: x, y backward (sign +1), then the Hermitian z line back to real.
@@ -526,21 +617,28 @@ void orc_fft_c2r_z(const orc_ctx *c, double *cw1d, double *rhs)
   const int nx = c->nx, ny = c->ny, nz = c->nz, nzh = nz / 2 + 1;
   cplx *cw1 = (cplx *)cw1d;
   int nmax = nx > ny ? nx : ny; nmax = nmax > nz ? nmax : nz;
-  cplx *work = malloc(sizeof(cplx) * 2 * nmax);
-  cplx *line = malloc(sizeof(cplx) * nz);
   cplx *twz = twiddles(nz, 1), *twy = twiddles(ny, 1), *twx = twiddles(nx, 1);
-  for (int k = 0; k < nzh; ++k)
-    for (int j = 0; j < ny; ++j) fft_line(cw1 + IDX(0, j, k, nx, ny), nx, 1, 1, work, twx);
-  for (int k = 0; k < nzh; ++k)
-    for (int i = 0; i < nx; ++i) fft_line(cw1 + IDX(i, 0, k, nx, ny), ny, nx, 1, work, twy);
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      for (int k = 0; k < nzh; ++k) line[k] = cw1[IDX(i, j, k, nx, ny)];
-      for (int k = nzh; k < nz; ++k) line[k] = conj(line[nz - k]);
-      fft_line(line, nz, 1, 1, work, twz);
-      for (int k = 0; k < nz; ++k) rhs[IDX(i, j, k, nx, ny)] = creal(line[k]);
-    }
-  free(work); free(line); free(twz); free(twy); free(twx);
+#pragma omp parallel
+  {
+    cplx *work = malloc(sizeof(cplx) * 2 * nmax);
+    cplx *line = malloc(sizeof(cplx) * nz);
+#pragma omp for schedule(static)
+    for (int k = 0; k < nzh; ++k)
+      for (int j = 0; j < ny; ++j) fft_line(cw1 + IDX(0, j, k, nx, ny), nx, 1, 1, work, twx);
+#pragma omp for schedule(static)
+    for (int k = 0; k < nzh; ++k)
+      for (int i = 0; i < nx; ++i) fft_line(cw1 + IDX(i, 0, k, nx, ny), ny, nx, 1, work, twy);
+#pragma omp for schedule(static)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        for (int k = 0; k < nzh; ++k) line[k] = cw1[IDX(i, j, k, nx, ny)];
+        for (int k = nzh; k < nz; ++k) line[k] = conj(line[nz - k]);
+        fft_line(line, nz, 1, 1, work, twz);
+        for (int k = 0; k < nz; ++k) rhs[IDX(i, j, k, nx, ny)] = creal(line[k]);
+      }
+    free(work); free(line);
+  }
+  free(twz); free(twy); free(twx);
 }
 
 /* The spectral kernel of poisson_000, statement by statement
@@ -550,6 +648,7 @@ void orc_poisson_spectral(const orc_ctx *c, double *cw1d)
 {
   const int nx = c->nx, ny = c->ny, nz = c->nz, nzh = nz / 2 + 1;
   const double ntot = (double)nx * (double)ny * (double)nz;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nzh; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -620,22 +719,30 @@ void orc_momentum_rhs(const orc_ctx *c, double *dux, double *duy, double *duz,
   for (int cpt = 0; cpt < 3; ++cpt) {
     const double *uc = u[cpt];
     /* x-pencil: ta = ux*uc; td = derx(ta); ta = derx(uc); du = td + ux*ta */
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) ta[q] = ux[q] * uc[q];
     orc_apply(c, td, ta, opx[cpt][0], 0, 0, nx, ny, nz);
     orc_apply(c, ta, uc, opx[cpt][1], 0, 0, nx, ny, nz);
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) du[cpt][q] = td[q] + ux[q] * ta[q];
     /* y-pencil: td = uc*uy; tg = dery(td); td = dery(uc); tg = tg + uy*td */
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) td[q] = cpt == 2 ? uz[q] * uy[q] : uc[q] * uy[q];
     orc_apply(c, tg, td, opy[cpt][0], 1, 0, nx, ny, nz);
     orc_apply(c, td, uc, opy[cpt][1], 1, 0, nx, ny, nz);
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) tg[q] = tg[q] + uy[q] * td[q];
     /* z-pencil: td = uc*uz; t3 = derz(td); td = derz(uc); t3 = t3 + uz*td */
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) td[q] = uc[q] * uz[q];
     orc_apply(c, t3, td, opz[cpt][0], 2, 0, nx, ny, nz);
     orc_apply(c, td, uc, opz[cpt][1], 2, 0, nx, ny, nz);
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) t3[q] = t3[q] + uz[q] * td[q];
     /* z->y: tg = tg + t3; y->x: du = du + tg */
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) tg[q] = tg[q] + t3[q];
+#pragma omp parallel for schedule(static)
     for (size_t q = 0; q < N; ++q) du[cpt][q] = du[cpt][q] + tg[q];
   }
   free(ta); free(td); free(tg); free(t3);
@@ -646,8 +753,11 @@ void orc_int_time(const orc_ctx *c, double gdt, double *ux, double *uy, double *
                   const double *dux, const double *duy, const double *duz)
 {
   const size_t N = (size_t)c->nx * c->ny * c->nz;
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < N; ++q) ux[q] = gdt * dux[q] + ux[q];
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < N; ++q) uy[q] = gdt * duy[q] + uy[q];
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < N; ++q) uz[q] = gdt * duz[q] + uz[q];
 }
 
@@ -664,10 +774,12 @@ void orc_divergence(const orc_ctx *c, double *pp3, const double *ux, const doubl
   orc_apply(c, pgz1, uz, ORC_OP_INTXVP, 0, 3, nx, ny, nz);
   orc_apply(c, upi2, pp1, ORC_OP_INTYVP, 1, 3, nx, ny, nz);
   orc_apply(c, duydypi2, pgy1, ORC_OP_DERYVP, 1, 1, nx, ny, nz);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < N; ++q) duydypi2[q] = duydypi2[q] + upi2[q];
   orc_apply(c, upi2, pgz1, ORC_OP_INTYVP, 1, 3, nx, ny, nz);
   orc_apply(c, pp3, duydypi2, ORC_OP_INTZVP, 2, 3, nx, ny, nz);
   orc_apply(c, po3, upi2, ORC_OP_DERZVP, 2, 1, nx, ny, nz);
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < N; ++q) pp3[q] = pp3[q] + po3[q];
   free(pp1); free(pgy1); free(pgz1); free(upi2); free(duydypi2); free(po3);
 }
@@ -693,6 +805,7 @@ void orc_cor_vel(const orc_ctx *c, double *ux, double *uy, double *uz,
                  const double *px, const double *py, const double *pz)
 {
   const size_t N = (size_t)c->nx * c->ny * c->nz;
+#pragma omp parallel for schedule(static)
   for (size_t q = 0; q < N; ++q) {
     ux[q] = ux[q] - px[q];
     uy[q] = uy[q] - py[q];
@@ -756,6 +869,7 @@ void orc_init_fields(const orc_ctx *c, int kind, uint64_t seed, double amp,
         ph[cpt][m] = twopi * orc_uniform(seed, 16 + cpt, (uint64_t)(4 * m + 3));
       }
   }
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
@@ -826,3 +940,7 @@ void orc_div_stats(const orc_ctx *c, const double *pp3, double *tmax, double *tm
   }
   *tmax = mx; *tmean = s / (double)N;
 }
+
+/* OpenMP threads used by the line loops (the results do not depend on it) */
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+int orc_max_threads(void) { return omp_get_max_threads(); }

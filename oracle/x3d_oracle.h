@@ -118,6 +118,10 @@ void orc_init_fields(const orc_ctx *c, int kind, uint64_t seed, double amp,
 double orc_test_err(const orc_ctx *c, const double *d, int which);
 void   orc_div_stats(const orc_ctx *c, const double *pp3, double *tmax, double *tmean);
 
+/* ---- threading ------------------------------------------------------------ */
+void orc_set_threads(int n);
+int  orc_max_threads(void);
+
 #ifdef __cplusplus
 }
 #endif
