@@ -87,7 +87,10 @@ int x3d_memcpy_d2h(x3d_ctx *ctx, void *dst, const void *src, size_t nbytes);
 int x3d_operator_get(const x3d_ctx *ctx, int op, double *f, double *s, double *w,
                      double *periodic, double *alfa, int *n, int *npaire);
 /* Replace one operator's tables (drop-in for x3d_operator_1d%init,
- * src/x3d_operator_1d.f90:234-266, when the caller builds its own). */
+ * src/x3d_operator_1d.f90:234-266, when the caller builds its own).  A step
+ * captured earlier (x3d_graph_capture_step*) is released: x3d_graph_launch
+ * then returns X3D_ERR_UNSUPPORTED, as does a new capture, since steps on
+ * caller tables run the unfused composition, which is not captured. */
 int x3d_operator_set(x3d_ctx *ctx, int op, const double *f, const double *s, const double *w,
                      const double *periodic, double alfa);
 /* Scheme constants per direction, 11 each: alfai afi bfi alcai aci bci ailcai aici bici cici dici

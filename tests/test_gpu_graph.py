@@ -45,3 +45,34 @@ def test_graph_replays_match_eager(torch_cuda, prec):
         ctx.derx(junk, eager[0]) if prec == "f64" else ctx.derx_f32(junk, eager[0])
         torch.cuda.synchronize()
     ctx.close()
+
+
+def test_operator_set_releases_captured_step(torch_cuda):
+    """Replacing an operator's tables replaces the device tables a captured
+    step reads: the graph is released and its launch refused.  Steps on
+    caller tables run the unfused composition, which is not captured; the
+    eager step with the caller's tables still matches the oracle."""
+    import torch
+    from x3div_amd import X3D, X3DError
+    nx, ny, nz = 32, 24, 16
+    ctx = X3D(nx, ny, nz, 1.0, 2.0, 3.0)
+    orc = Oracle(nx, ny, nz, 1.0, 2.0, 3.0)
+    u0 = orc.fields(1, 5, 1e-3)
+    f = [torch.from_numpy(a).cuda() for a in u0] + [ctx.field() for _ in range(7)]
+    g = [t.clone() for t in f]
+    ctx.graph_capture_step(*g, gdt=1e-3)
+    ctx.graph_launch()
+    torch.cuda.synchronize()
+    fs, s, w, p, alfa, _ = orc.op("derx")
+    ctx.set_operator("derx", fs, s, w, p, alfa)
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.graph_launch()
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.graph_capture_step(*g, gdt=1e-3)
+    ctx.step(*f, gdt=1e-3)
+    torch.cuda.synchronize()
+    ru, rdu, _, _ = orc.step(*u0, gdt=1e-3)
+    for a, b in zip(f[:6], ru + rdu):
+        a = a.cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+    ctx.close()

@@ -504,6 +504,9 @@ int x3d_operator_set(x3d_ctx* c, int op, const double* f, const double* s, const
   o.alfa = alfa;
   o.circulant = false;  // caller tables: always solved with the table sweeps
   X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  // a step captured earlier ran the circulant kernels on the old device
+  // tables, which upload_tables replaces: it must not be replayed
+  graph_release(c);
   return upload_tables(c);
 }
 
