@@ -233,6 +233,23 @@ int x3d_graph_capture_step_f32(x3d_ctx *ctx, double gdt, float *ux, float *uy, f
                                float *px, float *py, float *pz);
 int x3d_graph_launch(x3d_ctx *ctx);
 
+/* ---- the step without allocation ------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  x3d_step, bit for bit, with its 8 scratch fields taken
+ * from a static device array of the library (X3D_NOALLOC_MAX_POINTS * 8
+ * doubles, 128 MiB, which every device the library runs on holds until the
+ * process ends) instead of the context's own work fields, which x3d_step
+ * allocates on its first call.  It calls no hipMalloc or hipFree: the FFT
+ * plans, spectrum and tables it uses are made by x3d_create.  Grids of more
+ * than X3D_NOALLOC_MAX_POINTS points give X3D_ERR_SIZE.  All calls on one
+ * device share the array and it keeps the last call's contents: calls that
+ * overlap on one device (two streams, threads or contexts) race. */
+#define X3D_NOALLOC_MAX_POINTS (1 << 21) /* 128^3 */
+int x3d_step_noalloc(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
+                     double *dux, double *duy, double *duz, double *pp3,
+                     double *px, double *py, double *pz);
+
 /* ---- batching form ---------------------------------------------------------
  * Many independent triply periodic grids of uneven sizes, each the
  * reference's problem on its own (nx, ny, nz) (src/xcompact3d.f90:80-95),

@@ -68,6 +68,7 @@ SIGNATURES = {
     "x3d_cor_vel": (_i, [_vp] + [_dp] * 6),
     "x3d_solve_poisson": (_i, [_vp] + [_dp] * 7),
     "x3d_step": (_i, [_vp, _d] + [_dp] * 10),
+    "x3d_step_noalloc": (_i, [_vp, _d] + [_dp] * 10),
     "x3d_step_timed": (_i, [_vp, _d] + [_dp] * 10 + [C.POINTER(C.c_float)]),
     "x3d_derx_f32": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
     "x3d_dery_f32": (_i, [_vp, _dp, _dp, _i, _dp, _i, _i, _i]),

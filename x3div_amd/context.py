@@ -321,6 +321,12 @@ class X3D:
         check("x3d_step", self.lib.x3d_step(
             self._h, float(gdt), *(self._f(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
 
+    def step_noalloc(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """step() with its scratch in the library's static device array: no
+        device allocation (grids of at most X3D_NOALLOC_MAX_POINTS points)."""
+        check("x3d_step_noalloc", self.lib.x3d_step_noalloc(
+            self._h, float(gdt), *(self._f(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
+
     def step_timed(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
         """One step with a HIP event after each pass; waits for it and returns
         {pass name: (milliseconds, algorithmic HBM bytes per grid point)}."""

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 
 #include "x3d_internal.h"
 
@@ -360,6 +361,43 @@ double x3d_wtime(void)
   return (double)tv.tv_sec + 1e-6 * (double)tv.tv_usec;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of x3d_step_noalloc: the step's 8 work fields for grids of up to
+// X3D_NOALLOC_MAX_POINTS points, as a static device array.  Each device the
+// library is loaded on holds its own copy (128 MiB) until the process ends.
+// Host code never uses the array's name as an address: its device address is
+// looked up once per device with hipGetSymbolAddress (at x3d_create) and kept
+// in scratch_addr.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchElems = (size_t)8 * X3D_NOALLOC_MAX_POINTS;
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchElems];
+
+// Device address of `scratch` on the current device `dev`, looked up once.
+static int scratch_setup(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) return X3D_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_addr[dev]) return X3D_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return X3D_ERR_HIP;
+  scratch_addr[dev] = static_cast<double*>(p);
+  return X3D_OK;
+}
+
+static double* scratch_of(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_addr[dev];
+}
+
 int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_ctx** out)
 {
   if (!out) return X3D_ERR_ARG;
@@ -377,6 +415,7 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   c->dx = xlx / (double)nx; c->dy = yly / (double)ny; c->dz = zlz / (double)nz;  // nXm = nX
   int rc = X3D_OK;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return X3D_ERR_HIP; }
+  if ((rc = scratch_setup(c->device)) != X3D_OK) { delete c; return rc; }
   scheme(c->sc[0], c->dx);
   scheme(c->sc[1], c->dy);
   scheme(c->sc[2], c->dz);
@@ -949,6 +988,30 @@ int x3d_graph_launch(x3d_ctx* c)
   if (!c->graph_exec) return X3D_ERR_UNSUPPORTED;
   X3D_CHECK_HIP(hipGraphLaunch(c->graph_exec, c->stream));
   return X3D_OK;
+}
+
+// x3d_step with the 8 work fields in `scratch` (at its device address from
+// scratch_setup) instead of the context's own, which are swapped back after
+// the call.  The context's FFT plans, spectrum and tables come from
+// x3d_create, so nothing here allocates.  Every work field is written by a
+// pass before a later pass reads it, so the array needs no clearing.
+int x3d_step_noalloc(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double* dux, double* duy,
+                     double* duz, double* pp3, double* px, double* py, double* pz)
+{
+  if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  const size_t N = (size_t)c->nx * c->ny * c->nz;
+  if (N > (size_t)X3D_NOALLOC_MAX_POINTS) return X3D_ERR_SIZE;
+  double* base = scratch_of(c->device);
+  if (!base) return X3D_ERR_HIP;
+  double* own[8];
+  std::memcpy(own, c->work, sizeof(own));
+  const size_t own_elems = c->work_elems;
+  for (int i = 0; i < 8; ++i) c->work[i] = base + (size_t)i * N;
+  c->work_elems = N;
+  const int rc = x3d_step(c, gdt, ux, uy, uz, dux, duy, duz, pp3, px, py, pz);
+  std::memcpy(c->work, own, sizeof(own));
+  c->work_elems = own_elems;
+  return rc;
 }
 
 const char* x3d_pass_name(int i) { return (i >= 0 && i < X3D_NPASS) ? kPass[i].name : nullptr; }
