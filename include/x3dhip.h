@@ -81,6 +81,10 @@ int x3d_field_alloc(x3d_ctx *ctx, size_t nbytes, void **dptr);
 int x3d_field_free(x3d_ctx *ctx, void *dptr);
 int x3d_memcpy_h2d(x3d_ctx *ctx, void *dst, const void *src, size_t nbytes);
 int x3d_memcpy_d2h(x3d_ctx *ctx, void *dst, const void *src, size_t nbytes);
+/* n FP64 values of a device array set to v (a kernel, not a memset, so it
+ * can be recorded in a graph); the pressure and gradient zeroing of case's
+ * init (src/case.f90:36-44). */
+int x3d_fill(x3d_ctx *ctx, double *p, size_t n, double v);
 
 /* ---- operator tables (type x3doperator1d, src/x3d_operator_1d.f90:12-30) -- */
 /* Host copies of one operator's f, s, w, periodic vectors (length n) and alfa. */
@@ -103,6 +107,20 @@ int x3d_scheme_consts(const x3d_ctx *ctx, double *out33);
 int x3d_xthomas(x3d_ctx *ctx, double *tt, int op, int nx, int ny, int nz);
 int x3d_ythomas(x3d_ctx *ctx, double *tt, int op, int nx, int ny, int nz);
 int x3d_zthomas(x3d_ctx *ctx, double *tt, int op, int nx, int ny, int nz);
+/* The same generics called with the caller's own coefficient vectors, the
+ * reference's argument lists: xthomas_0(tt, ff, fs, fw, perio, alfa, ...) and,
+ * with perio = NULL, xthomas_12(tt, ff, fs, fw, ...) (src/thomas.f90:34-84,
+ * 87-136, 139-188).  ff, fs, fw, perio are host arrays of the line length,
+ * copied to the device for the call (the call waits for that copy). */
+int x3d_xthomas_tab(x3d_ctx *ctx, double *tt, const double *ff, const double *fs, const double *fw,
+                    const double *perio, double alfa, int nx, int ny, int nz);
+int x3d_ythomas_tab(x3d_ctx *ctx, double *tt, const double *ff, const double *fs, const double *fw,
+                    const double *perio, double alfa, int nx, int ny, int nz);
+int x3d_zthomas_tab(x3d_ctx *ctx, double *tt, const double *ff, const double *fs, const double *fw,
+                    const double *perio, double alfa, int nx, int ny, int nz);
+/* thomas1d (src/thomas.f90:200-218): one tridiagonal line on the host, in
+ * place, as x3doperator1d%init uses it for the periodic vector. */
+int x3d_thomas1d(double *tt, const double *ff, const double *fs, const double *fw, int n);
 
 /* ---- collocated first derivatives (procedure pointers derx/dery/derz,
  *      abstract interfaces src/x3d_derive.f90:18-51; periodic derx_00,
@@ -232,6 +250,15 @@ int x3d_graph_capture_step_f32(x3d_ctx *ctx, double gdt, float *ux, float *uy, f
                                float *dux, float *duy, float *duz, float *pp3,
                                float *px, float *py, float *pz);
 int x3d_graph_launch(x3d_ctx *ctx);
+/* Record any sequence of this context's calls into its graph instead: the
+ * calls between x3d_graph_begin and x3d_graph_end are recorded, not run
+ * (the reference-shaped driver records its transeq, int_time, solve_poisson
+ * and cor_vel calls, src/xcompact3d.f90:80-100, once, then replays them per
+ * step).  The FP64 step's scratch fields are allocated by x3d_graph_begin;
+ * recorded calls must not synchronise (reductions, x3d_sync), and a call
+ * that fails makes x3d_graph_end fail and discards the graph. */
+int x3d_graph_begin(x3d_ctx *ctx);
+int x3d_graph_end(x3d_ctx *ctx);
 
 /* ---- the step without allocation ------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

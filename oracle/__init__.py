@@ -57,6 +57,8 @@ def lib(fast: bool = False):
         L.orc_apply.restype = C.c_int
         L.orc_thomas.argtypes = [vp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_thomas.restype = C.c_int
+        for d in "xyz":
+            getattr(L, f"orc_{d}thomas_12").argtypes = [dp, dp, dp, dp, C.c_int, C.c_int, C.c_int]
         L.orc_prepare.argtypes = [dp, dp, dp, dp, dp, C.c_int]
         L.orc_thomas1d.argtypes = [dp, dp, dp, dp, C.c_int]
         L.orc_fft_r2c_z.argtypes = [vp, dp, dp]
@@ -160,6 +162,15 @@ class Oracle:
         tt = np.array(tt, dtype=np.float64, order="C", copy=True)
         rc = self._L.orc_thomas(self._c, _p(tt), OP[name], d, self.nx, self.ny, self.nz)
         assert rc == 0
+        return tt
+
+    def thomas12(self, name: str, tt: np.ndarray) -> np.ndarray:
+        """The plain sweeps with an operator's f, s, w (xthomas_12 and its y, z
+        forms), without the periodic correction."""
+        d = _dir_letter(name)
+        f, s, w, _, _, _ = self.op(name)
+        tt = np.array(tt, dtype=np.float64, order="C", copy=True)
+        getattr(self._L, f"orc_{d}thomas_12")(_p(tt), _p(f), _p(s), _p(w), self.nx, self.ny, self.nz)
         return tt
 
     def r2c(self, rhs):

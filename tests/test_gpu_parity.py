@@ -107,6 +107,23 @@ def test_thomas_parity(grid, axis, kind):
     assert relerr(host(t), ref) < RTOL
 
 
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+def test_thomas_with_caller_tables(grid, axis):
+    """The Thomas generics called with the caller's own vectors
+    (src/thomas.f90:15-28): the periodic member with perio and alfa, and the
+    plain sweeps (the _12 member) without."""
+    ctx, orc = grid
+    tt = rand(orc, 9)
+    name = f"int{axis}vp"
+    f, s, w, p, alfa, _ = orc.op(name)
+    t = dev(tt)
+    getattr(ctx, f"{axis}thomas_tab")(t, f, s, w, p, alfa)
+    assert relerr(host(t), orc.thomas(name, tt)) < RTOL
+    t = dev(tt)
+    getattr(ctx, f"{axis}thomas_tab")(t, f, s, w)
+    assert relerr(host(t), orc.thomas12(name, tt)) < RTOL
+
+
 def test_transposes_bit_exact(grid):
     ctx, orc = grid
     import torch

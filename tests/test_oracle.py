@@ -124,6 +124,34 @@ def test_periodic_thomas_is_the_circulant_solve(n):
         assert rel(got, ref) < 1e-13, name
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_plain_sweeps_solve_the_factored_tridiagonal(axis):
+    """xthomas_12 / ythomas_12 / zthomas_12 (src/thomas.f90:61-84, 114-136,
+    166-188): with an operator's f, s, w they solve, line by line, the
+    tridiagonal system those vectors factor.  prepare() keeps upper f,
+    multipliers s_i = b_{i-1}/p_{i-1} and inverted pivots w_i = 1/p_i with
+    p_i = c_i - f_{i-1} s_i, so b and c follow back from f, s, w."""
+    dims = [8, 8, 8]
+    dims[axis] = 24
+    o = Oracle(*dims, 1.0, 2.0, 3.0)
+    name = ["derx", "derypv", "intzvp"][axis]
+    f, s, w, _, _, _ = o.op(name)
+    n = dims[axis]
+    piv = 1.0 / w
+    c = piv.copy()
+    b = np.zeros(n)
+    for i in range(1, n):
+        b[i - 1] = s[i] * piv[i - 1]
+        c[i] = piv[i] + f[i - 1] * s[i]
+    A = np.diag(c) + np.diag(f[:-1], 1) + np.diag(b[:-1], -1)
+    r = np.random.default_rng(axis).standard_normal(o.shape)
+    x = o.thomas12(name, r)
+    # lines along the axis: move it last, then A acts on the last index
+    xl = np.moveaxis(x, 2 - axis, -1)
+    rl = np.moveaxis(r, 2 - axis, -1)
+    assert rel(xl @ A.T, rl) < 1e-13
+
+
 def test_operator_tables_equal_between_pairs(orc):
     # derX and derXp, intXvp and intXpv, derXvp and derXpv share tables (periodic)
     for a, b in (("derx", "derxp"), ("intxvp", "intxpv"), ("derxvp", "derxpv"), ("derz", "derzp")):

@@ -69,6 +69,13 @@ SIGNATURES = {
     "x3d_solve_poisson": (_i, [_vp] + [_dp] * 7),
     "x3d_step": (_i, [_vp, _d] + [_dp] * 10),
     "x3d_step_noalloc": (_i, [_vp, _d] + [_dp] * 10),
+    "x3d_graph_begin": (_i, [_vp]),
+    "x3d_graph_end": (_i, [_vp]),
+    "x3d_fill": (_i, [_vp, _dp, C.c_size_t, _d]),
+    "x3d_xthomas_tab": (_i, [_vp, _dp, _hp, _hp, _hp, _hp, _d, _i, _i, _i]),
+    "x3d_ythomas_tab": (_i, [_vp, _dp, _hp, _hp, _hp, _hp, _d, _i, _i, _i]),
+    "x3d_zthomas_tab": (_i, [_vp, _dp, _hp, _hp, _hp, _hp, _d, _i, _i, _i]),
+    "x3d_thomas1d": (_i, [_hp, _hp, _hp, _hp, _i]),
     "x3d_step_timed": (_i, [_vp, _d] + [_dp] * 10 + [C.POINTER(C.c_float)]),
     "x3d_derx_f32": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
     "x3d_dery_f32": (_i, [_vp, _dp, _dp, _i, _dp, _i, _i, _i]),

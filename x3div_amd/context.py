@@ -145,6 +145,25 @@ class X3D:
     def zthomas(self, tt, op):
         check("x3d_zthomas", self.lib.x3d_zthomas(self._h, self._f(tt), _op_id(op), self.nx, self.ny, self.nz))
 
+    def _thomas_tab(self, fn, tt, f, s, w, periodic, alfa):
+        hp = C.POINTER(C.c_double)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (f, s, w)]
+        per = None if periodic is None else np.ascontiguousarray(periodic, dtype=np.float64)
+        check(fn, getattr(self.lib, fn)(self._h, self._f(tt), *(a.ctypes.data_as(hp) for a in arrs),
+                                        None if per is None else per.ctypes.data_as(hp), float(alfa),
+                                        self.nx, self.ny, self.nz))
+
+    def xthomas_tab(self, tt, f, s, w, periodic=None, alfa=0.0):
+        """xthomas with the caller's own vectors: xthomas_0 (periodic given)
+        or xthomas_12 (periodic None), src/thomas.f90:34-84."""
+        self._thomas_tab("x3d_xthomas_tab", tt, f, s, w, periodic, alfa)
+
+    def ythomas_tab(self, tt, f, s, w, periodic=None, alfa=0.0):
+        self._thomas_tab("x3d_ythomas_tab", tt, f, s, w, periodic, alfa)
+
+    def zthomas_tab(self, tt, f, s, w, periodic=None, alfa=0.0):
+        self._thomas_tab("x3d_zthomas_tab", tt, f, s, w, periodic, alfa)
+
     # ---- derivatives (src/x3d_derive.f90) ----------------------------------------
     def derx(self, tx, ux, op="derx"):
         check("x3d_derx", self.lib.x3d_derx(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.ny, self.nz))
@@ -241,6 +260,17 @@ class X3D:
     def graph_launch(self):
         """Replay the captured step on the context's stream."""
         check("x3d_graph_launch", self.lib.x3d_graph_launch(self._h))
+
+    def graph_begin(self):
+        """Record this context's following calls into its graph (not run)."""
+        check("x3d_graph_begin", self.lib.x3d_graph_begin(self._h))
+
+    def graph_end(self):
+        check("x3d_graph_end", self.lib.x3d_graph_end(self._h))
+
+    def fill(self, t, value: float):
+        """Set every value of an FP64 device field (a kernel, recordable in a graph)."""
+        check("x3d_fill", self.lib.x3d_fill(self._h, _ptr(t), t.numel(), float(value)))
 
     # ---- precision form (FP32 fields) ------------------------------------------------
     def derx_f32(self, tx, ux, op="derx"):

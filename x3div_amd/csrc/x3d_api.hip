@@ -3,6 +3,7 @@
 // test-mode reductions.
 #include <sys/time.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -18,6 +19,8 @@ int hip_status(hipError_t e)
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return X3D_ERR_ALLOC;
   return X3D_ERR_HIP;
 }
+
+static int hip_status_or_ok(hipError_t e) { return e == hipSuccess ? X3D_OK : hip_status(e); }
 
 // ---------------------------------------------------------------------------
 // Operator coefficients (src/x3d_operator_1d.f90).
@@ -64,6 +67,7 @@ static void op_init(OpHost& o, const std::vector<double>& f, const std::vector<d
   for (int k = 1; k < n; ++k) o.p[k] = o.p[k] - o.p[k - 1] * s[k];
   o.p[n - 1] = o.p[n - 1] * w[n - 1];
   for (int k = n - 2; k >= 0; --k) o.p[k] = (o.p[k] - f[k] * o.p[k + 1]) * w[k];
+  o.f0 = o.f; o.s0 = o.s; o.w0 = o.w; o.p0 = o.p; o.alfa0 = alfa;
 }
 
 static void scheme(Scheme& s, double h)
@@ -434,7 +438,8 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
       (rc = line_plan(nz, true, &c->plan[2])) || (rc = line_plan_fused(nx, false, &c->fplan[0], variant)) ||
       (rc = line_plan_fused(ny, true, &c->fplan[1], variant)) ||
       (rc = line_plan_fused(nz, true, &c->fplan[2], variant)) ||
-      (rc = upload_tables(c)) || (rc = poisson_setup(c))) {
+      (rc = upload_tables(c)) || (rc = poisson_setup(c)) ||
+      (rc = hip_status_or_ok(hipMalloc(&c->usertab, 4 * sizeof(double) * (size_t)std::max(nx, std::max(ny, nz)))))) {
     x3d_destroy(c);
     return rc;
   }
@@ -458,6 +463,13 @@ int x3d_destroy(x3d_ctx* c)
   graph_release(c);
   for (void* t : c->circ_tab)
     if (t) (void)hipFree(t);
+  if (c->usertab) (void)hipFree(c->usertab);
+  if (c->cap_stream) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->cap_stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(c->cap_stream);
+  }
   delete c;
   return X3D_OK;
 }
@@ -539,9 +551,15 @@ int x3d_operator_set(x3d_ctx* c, int op, const double* f, const double* s, const
   if (!c || op < 0 || op >= X3D_NOPS || !f || !s || !w || !p) return X3D_ERR_ARG;
   OpHost& o = c->op[op];
   const int n = o.n;
+  const size_t nb = (size_t)n * sizeof(double);
+  // the library's own tables (e.g. a caller rebuilding them with
+  // x3doperator1d%init): the operator keeps the circulant solver
+  const bool own = alfa == o.alfa0 && !std::memcmp(f, o.f0.data(), nb) && !std::memcmp(s, o.s0.data(), nb) &&
+                   !std::memcmp(w, o.w0.data(), nb) && !std::memcmp(p, o.p0.data(), nb);
+  if (own && o.circulant) return X3D_OK;
   o.f.assign(f, f + n); o.s.assign(s, s + n); o.w.assign(w, w + n); o.p.assign(p, p + n);
   o.alfa = alfa;
-  o.circulant = false;  // caller tables: always solved with the table sweeps
+  o.circulant = own;  // caller tables: solved with the table sweeps
   X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
   // a step captured earlier ran the circulant kernels on the old device
   // tables, which upload_tables replaces: it must not be replayed
@@ -582,6 +600,70 @@ int x3d_zthomas(x3d_ctx* c, double* tt, int op, int nx, int ny, int nz)
 {
   int rc = check3(c, nx, ny, nz);
   return rc ? rc : launch_line(c, 2, K_NONE, op, tt, tt, true);
+}
+
+// The generics' members with the caller's own host tables: copied into the
+// context's device table (ordered on the stream after earlier calls that
+// read it), then the sweeps, and the periodic correction when perio is given.
+static int thomas_tab(x3d_ctx* c, int dir, double* tt, const double* ff, const double* fs, const double* fw,
+                      const double* perio, double alfa, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  if (rc) return rc;
+  if (!tt || !ff || !fs || !fw || !c->usertab) return X3D_ERR_ARG;
+  const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+  const size_t nb = (size_t)n * sizeof(double);
+  std::vector<double> h(4 * (size_t)n, 0.0);
+  std::memcpy(h.data(), ff, nb);
+  std::memcpy(h.data() + n, fs, nb);
+  std::memcpy(h.data() + 2 * n, fw, nb);
+  if (perio) std::memcpy(h.data() + 3 * n, perio, nb);
+  X3D_CHECK_HIP(hipMemcpyAsync(c->usertab, h.data(), 4 * nb, hipMemcpyHostToDevice, c->stream));
+  // the host staging vector must outlive the copy
+  X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return launch_thomas_tables(c, dir, c->usertab, alfa, tt, perio != nullptr);
+}
+
+int x3d_xthomas_tab(x3d_ctx* c, double* tt, const double* ff, const double* fs, const double* fw,
+                    const double* perio, double alfa, int nx, int ny, int nz)
+{
+  return c ? thomas_tab(c, 0, tt, ff, fs, fw, perio, alfa, nx, ny, nz) : X3D_ERR_ARG;
+}
+int x3d_ythomas_tab(x3d_ctx* c, double* tt, const double* ff, const double* fs, const double* fw,
+                    const double* perio, double alfa, int nx, int ny, int nz)
+{
+  return c ? thomas_tab(c, 1, tt, ff, fs, fw, perio, alfa, nx, ny, nz) : X3D_ERR_ARG;
+}
+int x3d_zthomas_tab(x3d_ctx* c, double* tt, const double* ff, const double* fs, const double* fw,
+                    const double* perio, double alfa, int nx, int ny, int nz)
+{
+  return c ? thomas_tab(c, 2, tt, ff, fs, fw, perio, alfa, nx, ny, nz) : X3D_ERR_ARG;
+}
+
+__global__ void k_fill(double* __restrict__ p, size_t n, double v)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+int x3d_fill(x3d_ctx* c, double* p, size_t n, double v)
+{
+  if (!c || (!p && n)) return X3D_ERR_ARG;
+  if (!n) return X3D_OK;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, c->stream, p, n, v);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+// thomas1d (src/thomas.f90:200-218): one line on the host.
+int x3d_thomas1d(double* tt, const double* ff, const double* fs, const double* fw, int n)
+{
+  if (!tt || !ff || !fs || !fw || n < 1) return X3D_ERR_ARG;
+  for (int k = 1; k < n; ++k) tt[k] = tt[k] - tt[k - 1] * fs[k];
+  tt[n - 1] = tt[n - 1] * fw[n - 1];
+  for (int k = n - 2; k >= 0; --k) tt[k] = (tt[k] - ff[k] * tt[k + 1]) * fw[k];
+  return X3D_OK;
 }
 
 // ---- collocated derivatives ----------------------------------------------
@@ -927,30 +1009,52 @@ static int fft_streams(x3d_ctx* c, hipStream_t s)
 
 // Record body's launches on a private stream into a graph; every allocation
 // (scratch fields, FFT plans) is made before the capture starts.
-static int capture_graph(x3d_ctx* c, const std::function<int()>& body)
+// Start recording the context's launches on a private capture stream (the
+// caller's stream is put back at the end).  Every allocation the recorded
+// calls need (the FP64 step's scratch fields, FFT plans) exists by now.
+static int graph_begin(x3d_ctx* c)
 {
+  if (c->cap_stream) return X3D_ERR_ARG;  // already capturing
   graph_release(c);
+  X3D_TRY(ensure_work(c));
   hipStream_t cs = nullptr;
   X3D_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  hipStream_t user = c->stream;
+  const hipStream_t user = c->stream;
   c->stream = cs;
   int rc = fft_streams(c, cs);
-  hipGraph_t g = nullptr;
-  if (!rc) {
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      rc = X3D_ERR_HIP;
-    } else {
-      rc = body();
-      const hipError_t e = hipStreamEndCapture(cs, &g);
-      if (!rc && e != hipSuccess) rc = hip_status(e);
-    }
+  if (!rc && hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = X3D_ERR_HIP;
+  if (rc) {
+    c->stream = user;
+    (void)fft_streams(c, user);
+    (void)hipStreamDestroy(cs);
+    return rc;
   }
-  c->stream = user;
-  const int rs = fft_streams(c, user);
+  c->cap_stream = cs;
+  c->cap_user = user;
+  return X3D_OK;
+}
+
+// Stop recording and instantiate the graph; body_rc is the status of the
+// recorded calls (a failed one discards the graph).
+static int graph_end(x3d_ctx* c, int body_rc)
+{
+  if (!c->cap_stream) return X3D_ERR_ARG;
+  hipStream_t cs = c->cap_stream;
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(cs, &g);
+  int rc = body_rc;
+  if (!rc && ee != hipSuccess) rc = hip_status(ee);
+  c->stream = c->cap_user;
+  const int rs = fft_streams(c, c->cap_user);
   (void)hipStreamDestroy(cs);
+  c->cap_stream = nullptr;
+  c->cap_user = nullptr;
   if (!rc) rc = rs;
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
+    // a call refused during the capture left its error as the runtime's last
+    // error; clear it so that the next launch check does not report it
+    (void)hipGetLastError();
     return rc;
   }
   const hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
@@ -961,6 +1065,24 @@ static int capture_graph(x3d_ctx* c, const std::function<int()>& body)
   }
   c->graph = g;
   return X3D_OK;
+}
+
+static int capture_graph(x3d_ctx* c, const std::function<int()>& body)
+{
+  X3D_TRY(graph_begin(c));
+  return graph_end(c, body());
+}
+
+int x3d_graph_begin(x3d_ctx* c)
+{
+  if (!c) return X3D_ERR_ARG;
+  return graph_begin(c);
+}
+
+int x3d_graph_end(x3d_ctx* c)
+{
+  if (!c) return X3D_ERR_ARG;
+  return graph_end(c, X3D_OK);
 }
 
 int x3d_graph_capture_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double* dux,

@@ -291,6 +291,27 @@ int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double*
   return e == hipSuccess ? X3D_OK : hip_status(e);
 }
 
+int launch_thomas_tables(x3d_ctx* c, int dir, const double* tab, double alfa, double* tt, bool periodic_fix)
+{
+  if (dir < 0 || dir > 2 || !tab || !tt) return X3D_ERR_ARG;
+  if (!c->f64_lines) return X3D_ERR_UNSUPPORTED;
+  const LinePlan& P = c->plan[dir];
+  const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
+  if (P.n != n) return X3D_ERR_SIZE;
+  if (P.M == 0) return X3D_ERR_UNSUPPORTED;
+  Geom geo;
+  Tiling tl;
+  dim3 grid, block;
+  size_t smem;
+  launch_dims(c, dir, &geo, &tl, &grid, &block, &smem, sizeof(double));
+  tl.fix = periodic_fix ? 1 : 0;
+  const OpDev<double> od{tab, tab + n, tab + 2 * n, tab + 3 * n, alfa};
+  const Coef<double> cf{};
+  const Circ<double> cc{};
+  const hipError_t e = launch_kind<double, K_NONE, false>(P.M, grid, block, smem, c->stream, tt, tt, od, cf, cc, geo, tl);
+  return e == hipSuccess ? X3D_OK : hip_status(e);
+}
+
 // FP32 precision form of one operator: the circulant solver in FP32 (the
 // library's periodic tables only; no Thomas-table path).
 int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out)

@@ -35,6 +35,8 @@ struct OpHost {
   double alfa = 0.0;
   bool circulant = false;  // tables are the library's periodic [alfa, 1, alfa] ones
   std::vector<double> f, s, w, p;
+  std::vector<double> f0, s0, w0, p0;  // the library's own tables, kept to recognise them
+  double alfa0 = 0.0;
 };
 
 // Device view of one operator: four length-n vectors and alfa.
@@ -102,9 +104,14 @@ struct x3d_ctx {
   // dense-circulant fragments of the matrix-core variant, per operator (made on first use)
   void* circ_tab[X3D_NOPS] = {};
   float circ_scale[X3D_NOPS] = {};  // 2^-s of the power-of-two scaling of each table
-  // captured step (x3d_graph_capture_step*), replayed by x3d_graph_launch
+  // captured step (x3d_graph_capture_step*, or the calls between
+  // x3d_graph_begin and x3d_graph_end), replayed by x3d_graph_launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  hipStream_t cap_stream = nullptr;  // private capture stream while capturing
+  hipStream_t cap_user = nullptr;    // the caller's stream, restored at the end
+  // device copy of caller-supplied Thomas tables (f, s, w, periodic; 4 * max(nx, ny, nz))
+  double* usertab = nullptr;
 };
 
 // A batch of independent grids (the batching form).  Each grid keeps its own
@@ -160,6 +167,9 @@ int line_plan_fused(int n, bool line_fast, LinePlan* out, int variant);
 int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out);
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
                 bool periodic_fix);
+// The Thomas sweeps alone (in place) with device tables f, s, w, p at
+// tab + {0, 1, 2, 3} * n; periodic_fix adds the periodic correction.
+int launch_thomas_tables(x3d_ctx* c, int dir, const double* tab, double alfa, double* tt, bool periodic_fix);
 
 struct Geom;
 struct Tiling;

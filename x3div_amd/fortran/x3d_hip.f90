@@ -1,8 +1,7 @@
-!! iso_c_binding interface to libx3dhip (include/x3dhip.h) and the modules
-!! that keep x3div's public names and call signatures for the hot path:
-!!   decomp_2d_constants, x3d_hip, x3d_operator_1d, thomas, x3d_derive,
-!!   x3d_staggered, x3d_transpose, decomp_2d_poisson, transeq,
-!!   time_integrators, navier.
+!! iso_c_binding interface to libx3dhip (include/x3dhip.h): modules
+!! decomp_2d_constants and x3d_hip.  The modules that keep x3div's public
+!! names and call signatures are built on it in decomp_2d.f90,
+!! x3d_hip_ops.f90 and x3d_hip_step.f90.
 !! Every field argument is a device array (allocated through x3d_field_alloc
 !! and mapped with c_f_pointer); the host never dereferences it.  A nonzero
 !! status from the library stops the program the way decomp_2d_abort does.
@@ -47,6 +46,16 @@ module x3d_hip
     integer(c_int) function x3d_field_free(ctx, p) bind(C, name="x3d_field_free")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, p
+    end function
+    integer(c_int) function x3d_memcpy_d2h(ctx, dst, src, nbytes) bind(C, name="x3d_memcpy_d2h")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx, dst, src
+      integer(c_size_t), value :: nbytes
+    end function
+    integer(c_int) function x3d_memcpy_h2d(ctx, dst, src, nbytes) bind(C, name="x3d_memcpy_h2d")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx, dst, src
+      integer(c_size_t), value :: nbytes
     end function
     integer(c_int) function x3d_operator_get(ctx, op, f, s, w, p, alfa, n, npaire) bind(C, name="x3d_operator_get")
       import :: c_int, c_ptr, c_double
@@ -147,6 +156,62 @@ module x3d_hip
       integer(c_int64_t), value :: seed
       real(c_double), value :: amp
     end function
+    integer(c_int) function x3d_xthomas_tab(ctx, tt, ff, fs, fw, perio, alfa, nx, ny, nz) &
+        bind(C, name="x3d_xthomas_tab")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, tt, ff, fs, fw, perio
+      real(c_double), value :: alfa
+      integer(c_int), value :: nx, ny, nz
+    end function
+    integer(c_int) function x3d_ythomas_tab(ctx, tt, ff, fs, fw, perio, alfa, nx, ny, nz) &
+        bind(C, name="x3d_ythomas_tab")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, tt, ff, fs, fw, perio
+      real(c_double), value :: alfa
+      integer(c_int), value :: nx, ny, nz
+    end function
+    integer(c_int) function x3d_zthomas_tab(ctx, tt, ff, fs, fw, perio, alfa, nx, ny, nz) &
+        bind(C, name="x3d_zthomas_tab")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, tt, ff, fs, fw, perio
+      real(c_double), value :: alfa
+      integer(c_int), value :: nx, ny, nz
+    end function
+    integer(c_int) function x3d_thomas1d(tt, ff, fs, fw, n) bind(C, name="x3d_thomas1d")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tt, ff, fs, fw
+      integer(c_int), value :: n
+    end function
+    integer(c_int) function x3d_operator_set(ctx, op, f, s, w, p, alfa) bind(C, name="x3d_operator_set")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, f, s, w, p
+      integer(c_int), value :: op
+      real(c_double), value :: alfa
+    end function
+    integer(c_int) function x3d_copy_field(ctx, a, b, eb, n1, n2, n3) bind(C, name="x3d_transpose_x_to_y")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx, a, b
+      integer(c_size_t), value :: eb
+      integer(c_int), value :: n1, n2, n3
+    end function
+    integer(c_int) function x3d_fill(ctx, p, n, v) bind(C, name="x3d_fill")
+      import :: c_int, c_ptr, c_size_t, c_double
+      type(c_ptr), value :: ctx, p
+      integer(c_size_t), value :: n
+      real(c_double), value :: v
+    end function
+    integer(c_int) function x3d_graph_begin(ctx) bind(C, name="x3d_graph_begin")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function x3d_graph_end(ctx) bind(C, name="x3d_graph_end")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function x3d_graph_launch(ctx) bind(C, name="x3d_graph_launch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
     function x3d_strerror(code) bind(C, name="x3d_strerror") result(s)
       import :: c_int, c_ptr
       integer(c_int), value :: code
@@ -172,241 +237,3 @@ contains
 
 end module x3d_hip
 
-!! The x3doperator1d type and its 18 instances (src/x3d_operator_1d.f90:12-47).
-!! Components keep the reference's names; `id` is the library's operator id.
-module x3d_operator_1d
-  use, intrinsic :: iso_c_binding
-  use decomp_2d_constants, only : mytype
-  use x3d_hip
-  implicit none
-  private
-
-  type, public :: x3doperator1d
-    integer :: n = 0
-    integer :: npaire = 0
-    real(mytype) :: alfa = 0._mytype
-    real(mytype), dimension(:), pointer :: f => null(), s => null(), w => null()
-    real(mytype), dimension(:), allocatable :: periodic
-    integer :: id = -1
-  end type x3doperator1d
-
-  type(x3doperator1d), save, public, target :: x3d_op_derx, x3d_op_derxp
-  type(x3doperator1d), save, public, target :: x3d_op_dery, x3d_op_deryp
-  type(x3doperator1d), save, public, target :: x3d_op_derz, x3d_op_derzp
-  type(x3doperator1d), save, public, target :: x3d_op_derxvp, x3d_op_deryvp, x3d_op_derzvp
-  type(x3doperator1d), save, public, target :: x3d_op_derxpv, x3d_op_derypv, x3d_op_derzpv
-  type(x3doperator1d), save, public, target :: x3d_op_intxvp, x3d_op_intyvp, x3d_op_intzvp
-  type(x3doperator1d), save, public, target :: x3d_op_intxpv, x3d_op_intypv, x3d_op_intzpv
-
-  public :: x3d_operator_1d_init, x3d_operator_1d_finalize, x3d_op_by_id
-
-contains
-
-  !> x3d_operator_1d_init (src/x3d_operator_1d.f90:59-195): the coefficient
-  !> vectors come from the library, which builds them on context creation.
-  subroutine x3d_operator_1d_init()
-    call fill(x3d_op_derx, 0);   call fill(x3d_op_derxp, 1)
-    call fill(x3d_op_dery, 2);   call fill(x3d_op_deryp, 3)
-    call fill(x3d_op_derz, 4);   call fill(x3d_op_derzp, 5)
-    call fill(x3d_op_derxvp, 6); call fill(x3d_op_deryvp, 7);  call fill(x3d_op_derzvp, 8)
-    call fill(x3d_op_derxpv, 9); call fill(x3d_op_derypv, 10); call fill(x3d_op_derzpv, 11)
-    call fill(x3d_op_intxvp, 12); call fill(x3d_op_intyvp, 13); call fill(x3d_op_intzvp, 14)
-    call fill(x3d_op_intxpv, 15); call fill(x3d_op_intypv, 16); call fill(x3d_op_intzpv, 17)
-  end subroutine x3d_operator_1d_init
-
-  subroutine fill(op, id)
-    type(x3doperator1d), intent(inout) :: op
-    integer, intent(in) :: id
-    real(c_double), allocatable, target :: f(:), s(:), w(:), p(:)
-    real(c_double) :: alfa
-    integer(c_int) :: n, npaire
-    ! the length first (null vectors are not written), then the vectors
-    call x3d_check('x3d_operator_get', x3d_operator_get(x3d_ctx_h, int(id, c_int), c_null_ptr, c_null_ptr, &
-                                                         c_null_ptr, c_null_ptr, alfa, n, npaire))
-    allocate (f(n), s(n), w(n), p(n))
-    call x3d_check('x3d_operator_get', x3d_operator_get(x3d_ctx_h, int(id, c_int), c_loc(f), c_loc(s), &
-                                                         c_loc(w), c_loc(p), alfa, n, npaire))
-    op%n = n; op%npaire = npaire; op%alfa = alfa; op%id = id
-    allocate (op%f(n), op%s(n), op%w(n))
-    op%f = f; op%s = s; op%w = w
-    call move_alloc(p, op%periodic)
-  end subroutine fill
-
-  subroutine x3d_operator_1d_finalize()
-    call drop(x3d_op_derx); call drop(x3d_op_derxp); call drop(x3d_op_dery); call drop(x3d_op_deryp)
-    call drop(x3d_op_derz); call drop(x3d_op_derzp)
-    call drop(x3d_op_derxvp); call drop(x3d_op_deryvp); call drop(x3d_op_derzvp)
-    call drop(x3d_op_derxpv); call drop(x3d_op_derypv); call drop(x3d_op_derzpv)
-    call drop(x3d_op_intxvp); call drop(x3d_op_intyvp); call drop(x3d_op_intzvp)
-    call drop(x3d_op_intxpv); call drop(x3d_op_intypv); call drop(x3d_op_intzpv)
-  end subroutine x3d_operator_1d_finalize
-
-  subroutine drop(op)
-    type(x3doperator1d), intent(inout) :: op
-    if (associated(op%f)) deallocate (op%f, op%s, op%w)
-    if (allocated(op%periodic)) deallocate (op%periodic)
-    op%n = 0; op%npaire = 0; op%id = -1
-  end subroutine drop
-
-  !> The operator instance whose coefficient vector f is the given array
-  !> (lets the Thomas generics keep the reference's array arguments).
-  integer function x3d_op_by_id(ff) result(id)
-    real(mytype), dimension(:), target, intent(in) :: ff
-    type(c_ptr) :: a
-    a = c_loc(ff(1))
-    id = -1
-    if (c_associated(a, c_loc(x3d_op_derx%f(1)))) id = 0
-    if (c_associated(a, c_loc(x3d_op_derxp%f(1)))) id = 1
-    if (c_associated(a, c_loc(x3d_op_dery%f(1)))) id = 2
-    if (c_associated(a, c_loc(x3d_op_deryp%f(1)))) id = 3
-    if (c_associated(a, c_loc(x3d_op_derz%f(1)))) id = 4
-    if (c_associated(a, c_loc(x3d_op_derzp%f(1)))) id = 5
-    if (c_associated(a, c_loc(x3d_op_derxvp%f(1)))) id = 6
-    if (c_associated(a, c_loc(x3d_op_deryvp%f(1)))) id = 7
-    if (c_associated(a, c_loc(x3d_op_derzvp%f(1)))) id = 8
-    if (c_associated(a, c_loc(x3d_op_derxpv%f(1)))) id = 9
-    if (c_associated(a, c_loc(x3d_op_derypv%f(1)))) id = 10
-    if (c_associated(a, c_loc(x3d_op_derzpv%f(1)))) id = 11
-    if (c_associated(a, c_loc(x3d_op_intxvp%f(1)))) id = 12
-    if (c_associated(a, c_loc(x3d_op_intyvp%f(1)))) id = 13
-    if (c_associated(a, c_loc(x3d_op_intzvp%f(1)))) id = 14
-    if (c_associated(a, c_loc(x3d_op_intxpv%f(1)))) id = 15
-    if (c_associated(a, c_loc(x3d_op_intypv%f(1)))) id = 16
-    if (c_associated(a, c_loc(x3d_op_intzpv%f(1)))) id = 17
-  end function x3d_op_by_id
-
-end module x3d_operator_1d
-
-!! Thomas generics (src/thomas.f90:15-28) with the periodic variants
-!! xthomas_0/ythomas_0/zthomas_0 (:34-58, 87-111, 139-163).
-module thomas
-  use, intrinsic :: iso_c_binding
-  use decomp_2d_constants, only : mytype
-  use x3d_hip
-  use x3d_operator_1d, only : x3d_op_by_id
-  implicit none
-  private
-  public :: xthomas, ythomas, zthomas
-
-  interface xthomas
-    module procedure xthomas_0
-  end interface xthomas
-  interface ythomas
-    module procedure ythomas_0
-  end interface ythomas
-  interface zthomas
-    module procedure zthomas_0
-  end interface zthomas
-
-contains
-
-  subroutine xthomas_0(tt, ff, fs, fw, perio, alfa, nx, ny, nz)
-    integer, intent(in) :: nx, ny, nz
-    real(mytype), intent(inout), dimension(nx, ny, nz), target :: tt
-    real(mytype), intent(in), dimension(nx), target :: ff, fs, fw, perio
-    real(mytype), intent(in) :: alfa
-    call x3d_check('xthomas', x3d_xthomas(x3d_ctx_h, c_loc(tt), int(op_of(ff), c_int), nx, ny, nz))
-  end subroutine xthomas_0
-
-  subroutine ythomas_0(tt, ff, fs, fw, perio, alfa, nx, ny, nz)
-    integer, intent(in) :: nx, ny, nz
-    real(mytype), intent(inout), dimension(nx, ny, nz), target :: tt
-    real(mytype), intent(in), dimension(ny), target :: ff, fs, fw, perio
-    real(mytype), intent(in) :: alfa
-    call x3d_check('ythomas', x3d_ythomas(x3d_ctx_h, c_loc(tt), int(op_of(ff), c_int), nx, ny, nz))
-  end subroutine ythomas_0
-
-  subroutine zthomas_0(tt, ff, fs, fw, perio, alfa, nx, ny, nz)
-    integer, intent(in) :: nx, ny, nz
-    real(mytype), intent(inout), dimension(nx, ny, nz), target :: tt
-    real(mytype), intent(in), dimension(nz), target :: ff, fs, fw, perio
-    real(mytype), intent(in) :: alfa
-    call x3d_check('zthomas', x3d_zthomas(x3d_ctx_h, c_loc(tt), int(op_of(ff), c_int), nx, ny, nz))
-  end subroutine zthomas_0
-
-  integer function op_of(ff)
-    real(mytype), dimension(:), target, intent(in) :: ff
-    op_of = x3d_op_by_id(ff)
-    if (op_of < 0) call x3d_check('thomas: coefficient vectors of no known operator', -1_c_int)
-  end function op_of
-
-end module thomas
-
-!! Derivative procedure pointers (src/x3d_derive.f90:18-109), periodic derx_00,
-!! dery_00, derz_00 (:111-152, :279-339, :503-558).
-module x3d_derive
-  use, intrinsic :: iso_c_binding
-  use decomp_2d_constants, only : mytype
-  use x3d_hip
-  use x3d_operator_1d, only : x3doperator1d
-  implicit none
-  public
-
-  abstract interface
-    subroutine DERIVATIVE_X(t, u, x3dop, nx, ny, nz)
-      import :: mytype, x3doperator1d
-      integer, intent(in) :: nx, ny, nz
-      real(mytype), intent(out), dimension(nx, ny, nz), target :: t
-      real(mytype), intent(in), dimension(nx, ny, nz), target :: u
-      type(x3doperator1d), intent(in) :: x3dop
-    end subroutine DERIVATIVE_X
-    subroutine DERIVATIVE_Y(t, u, x3dop, pp, nx, ny, nz)
-      import :: mytype, x3doperator1d
-      integer, intent(in) :: nx, ny, nz
-      real(mytype), intent(out), dimension(nx, ny, nz), target :: t
-      real(mytype), intent(in), dimension(nx, ny, nz), target :: u
-      real(mytype), intent(in), dimension(ny) :: pp
-      type(x3doperator1d), intent(in) :: x3dop
-    end subroutine DERIVATIVE_Y
-    subroutine DERIVATIVE_Z(t, u, x3dop, nx, ny, nz)
-      import :: mytype, x3doperator1d
-      integer, intent(in) :: nx, ny, nz
-      real(mytype), intent(out), dimension(nx, ny, nz), target :: t
-      real(mytype), intent(in), dimension(nx, ny, nz), target :: u
-      type(x3doperator1d), intent(in) :: x3dop
-    end subroutine DERIVATIVE_Z
-  end interface
-
-  procedure(DERIVATIVE_X), pointer :: derx => null()
-  procedure(DERIVATIVE_Y), pointer :: dery => null()
-  procedure(DERIVATIVE_Z), pointer :: derz => null()
-
-contains
-
-  !> x3d_derive_init (src/x3d_derive.f90:66-93), periodic BCs.
-  subroutine x3d_derive_init()
-    derx => derx_00
-    dery => dery_00
-    derz => derz_00
-  end subroutine x3d_derive_init
-
-  subroutine x3d_derive_finalize()
-    nullify (derx, dery, derz)
-  end subroutine x3d_derive_finalize
-
-  subroutine derx_00(t, u, x3dop, nx, ny, nz)
-    integer, intent(in) :: nx, ny, nz
-    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
-    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
-    type(x3doperator1d), intent(in) :: x3dop
-    call x3d_check('derx', x3d_derx(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), nx, ny, nz))
-  end subroutine derx_00
-
-  subroutine dery_00(t, u, x3dop, pp, nx, ny, nz)
-    integer, intent(in) :: nx, ny, nz
-    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
-    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
-    real(mytype), intent(in), dimension(ny) :: pp
-    type(x3doperator1d), intent(in) :: x3dop
-    call x3d_check('dery', x3d_dery(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), c_null_ptr, nx, ny, nz))
-  end subroutine dery_00
-
-  subroutine derz_00(t, u, x3dop, nx, ny, nz)
-    integer, intent(in) :: nx, ny, nz
-    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
-    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
-    type(x3doperator1d), intent(in) :: x3dop
-    call x3d_check('derz', x3d_derz(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), nx, ny, nz))
-  end subroutine derz_00
-
-end module x3d_derive
