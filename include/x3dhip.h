@@ -45,7 +45,12 @@ enum {
   X3D_OP_DERXPV, X3D_OP_DERYPV, X3D_OP_DERZPV,
   X3D_OP_INTXVP, X3D_OP_INTYVP, X3D_OP_INTZVP,
   X3D_OP_INTXPV, X3D_OP_INTYPV, X3D_OP_INTZPV,
-  X3D_NOPS
+  X3D_NOPS,
+  /* Second derivatives of derxx / deryy / derzz (procedure pointers declared
+   * at src/x3d_derive.f90:54-59 and never associated there): this build's
+   * own operator instances, after the reference's 18. */
+  X3D_OP_DERXX = X3D_NOPS, X3D_OP_DERYY, X3D_OP_DERZZ,
+  X3D_NOPS_ALL
 };
 
 /* Initial-field kinds for x3d_init_fields. */
@@ -132,6 +137,19 @@ int x3d_dery(x3d_ctx *ctx, double *ty, const double *uy, int op, const double *p
              int nx, int ny, int nz);
 int x3d_derz(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nz);
 
+/* ---- second derivatives (procedure pointers derxx/deryy/derzz with the
+ *      DERIVATIVE_X / DERIVATIVE_YY / DERIVATIVE_Z interfaces,
+ *      src/x3d_derive.f90:36-59; the reference declares them and keeps the
+ *      coefficient names alsaix, asix, bsix, src/x3d_operator_x_data.f90:24,
+ *      but implements no scheme).  This build's scheme is the sixth-order
+ *      compact one of isecondder = 4 (src/parameters.f90:107; Lele 1992):
+ *      alfa t_{i-1} + t_i + alfa t_{i+1}
+ *        = a (u_{i+1} - 2u_i + u_{i-1}) / h^2 + b (u_{i+2} - 2u_i + u_{i-2}) / (4 h^2),
+ *      alfa = 2/11, a = 12/11, b = 3/11, periodic; op is X3D_OP_DERXX/YY/ZZ. */
+int x3d_derxx(x3d_ctx *ctx, double *tx, const double *ux, int op, int nx, int ny, int nz);
+int x3d_deryy(x3d_ctx *ctx, double *ty, const double *uy, int op, int nx, int ny, int nz);
+int x3d_derzz(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nz);
+
 /* ---- staggered derivatives and interpolations (src/x3d_staggered.f90) ---- */
 int x3d_derxvp(x3d_ctx *ctx, double *tx, const double *ux, int op, int nx, int nxm, int ny, int nz);    /* :23-106  */
 int x3d_interxvp(x3d_ctx *ctx, double *tx, const double *ux, int op, int nx, int nxm, int ny, int nz);  /* :110-217 */
@@ -216,9 +234,10 @@ double x3d_pass_bytes_per_point(int pass);
 int x3d_derx_f32(x3d_ctx *ctx, float *tx, const float *ux, int op, int nx, int ny, int nz);
 int x3d_dery_f32(x3d_ctx *ctx, float *ty, const float *uy, int op, const float *ppy, int nx, int ny, int nz);
 int x3d_derz_f32(x3d_ctx *ctx, float *tz, const float *uz, int op, int nx, int ny, int nz);
-/* Any of the 18 operators along its own direction: derivatives (op 0-5), the
- * staggered derivatives and interpolations (op 6-17), as the FP64 routines
- * of src/x3d_derive.f90 and src/x3d_staggered.f90 with periodic BCs. */
+/* Any operator along its own direction: derivatives (op 0-5), the staggered
+ * derivatives and interpolations (op 6-17), as the FP64 routines of
+ * src/x3d_derive.f90 and src/x3d_staggered.f90 with periodic BCs, and the
+ * second derivatives (X3D_OP_DERXX..ZZ). */
 int x3d_apply_f32(x3d_ctx *ctx, int op, float *out, const float *in);
 int x3d_poisson_f32(x3d_ctx *ctx, float *rhs);
 /* Matrix-core variant of x3d_apply_f32: with periodic boundaries each

@@ -23,9 +23,12 @@ FAST_LIB_PATH = os.path.join(HERE, "liborcx3d_fast.so")
 OPS = ["derx", "derxp", "dery", "deryp", "derz", "derzp",
        "derxvp", "deryvp", "derzvp", "derxpv", "derypv", "derzpv",
        "intxvp", "intyvp", "intzvp", "intxpv", "intypv", "intzpv"]
-OP = {name: i for i, name in enumerate(OPS)}
-# stencil kinds: D, Ds v->p, Ds p->v, I v->p, I p->v
-KIND = {"d": 0, "dvp": 1, "dpv": 2, "ivp": 3, "ipv": 4}
+# second derivatives after them (declared and never implemented by the
+# reference, src/x3d_derive.f90:54-59)
+OPS2 = ["derxx", "deryy", "derzz"]
+OP = {name: i for i, name in enumerate(OPS + OPS2)}
+# stencil kinds: D, Ds v->p, Ds p->v, I v->p, I p->v, D2
+KIND = {"d": 0, "dvp": 1, "dpv": 2, "ivp": 3, "ipv": 4, "dd": 5}
 
 
 def build(force: bool = False) -> str:
@@ -250,6 +253,8 @@ def _dir_letter(name: str) -> str:
 
 
 def _default_kind(name: str) -> str:
+    if name in OPS2:
+        return "dd"
     if name.startswith("der"):
         return "dvp" if name.endswith("vp") else "dpv" if name.endswith("pv") else "d"
     return "ivp" if name.endswith("vp") else "ipv"

@@ -90,6 +90,12 @@ static void scheme_consts(orc_ctx *c, int d, double h)
   c->bici[d] = 1.0 / 256.0 * (126.0 * c->ailcai[d] - 25.0);
   c->cici[d] = 1.0 / 256.0 * (-10.0 * c->ailcai[d] + 3.0);
   c->dici[d] = 0.0;
+  /* second derivative (isecondder = 4, src/parameters.f90:107; the reference
+   * implements none): Lele's (1992) sixth-order tridiagonal scheme,
+   * alfa = 2/11, a = 12/11 on (u+1 - 2u + u-1)/h^2, b = 3/11 on (u+2 - 2u + u-2)/(4h^2) */
+  c->alsai[d] = 2.0 / 11.0;
+  c->asi[d] = (12.0 / 11.0) / (h * h);
+  c->bsi[d] = (3.0 / 44.0) / (h * h);
 }
 
 /* One direction of x3d_operator_1d_init (src/x3d_operator_1d.f90:76-189):
@@ -130,6 +136,12 @@ static int dir_ops(orc_ctx *c, int d, int n)
   memset(sp, 0, sizeof(double) * n);
   orc_prepare(b, cc, fp, sp, wp, n);
   rc |= op_init(&c->ops[ORC_OP_INTXVP + d], fp, sp, wp, n, c->ailcai[d], 1);
+
+  /* DD: the same periodic rows with alfa = 2/11 */
+  periodic_rows(c->alsai[d], f, cc, b, n);
+  memset(s, 0, sizeof(double) * n);
+  orc_prepare(b, cc, f, s, w, n);
+  rc |= op_init(&c->ops[ORC_OP_DERXX + d], f, s, w, n, c->alsai[d], 0);
 out:
   free(f); free(cc); free(b); free(s); free(w); free(fp); free(sp); free(wp);
   return rc;
@@ -241,7 +253,7 @@ int orc_init(orc_ctx *c, int nx, int ny, int nz, double xlx, double yly, double 
 
 void orc_free(orc_ctx *c)
 {
-  for (int o = 0; o < ORC_NOPS; ++o) {
+  for (int o = 0; o < ORC_NOPS_ALL; ++o) {
     free(c->ops[o].f); free(c->ops[o].s); free(c->ops[o].w); free(c->ops[o].periodic);
   }
   free(c->ax); free(c->bx); free(c->ay); free(c->by); free(c->az); free(c->bz);
@@ -268,7 +280,7 @@ void orc_delete(orc_ctx *c)
 int orc_get_op(const orc_ctx *c, int op, double *f, double *s, double *w,
                double *periodic, double *alfa, int *n, int *npaire)
 {
-  if (op < 0 || op >= ORC_NOPS) return -1;
+  if (op < 0 || op >= ORC_NOPS_ALL) return -1;
   const orc_op *o = &c->ops[op];
   memcpy(f, o->f, sizeof(double) * o->n);
   memcpy(s, o->s, sizeof(double) * o->n);
@@ -405,7 +417,7 @@ void orc_zthomas_0(double *tt, const orc_op *op, int nx, int ny, int nz)
 
 int orc_thomas(const orc_ctx *c, double *tt, int op, int dir, int nx, int ny, int nz)
 {
-  if (op < 0 || op >= ORC_NOPS) return -1;
+  if (op < 0 || op >= ORC_NOPS_ALL) return -1;
   const orc_op *o = &c->ops[op];
   int nl = dir == 0 ? nx : dir == 1 ? ny : nz;
   if (o->n != nl) return -2;
@@ -439,6 +451,7 @@ static double rhs_point(const double *u, long stride, int i, int n, int kind,
   case 1: return a * (U(1) - U(0)) + b * (U(2) - U(-1));
   case 2: return a * (U(0) - U(-1)) + b * (U(1) - U(-2));
   case 3: return a * (U(1) + U(0)) + b * (U(2) + U(-1)) + cc * (U(3) + U(-2)) + d * (U(4) + U(-3));
+  case 5: return a * ((U(1) + U(-1)) - 2.0 * U(0)) + b * ((U(2) + U(-2)) - 2.0 * U(0));
   default: return a * (U(0) + U(-1)) + b * (U(1) + U(-2)) + cc * (U(2) + U(-3)) + d * (U(3) + U(-4));
   }
 #undef U
@@ -458,6 +471,10 @@ static void rhs_run(double *t, const double *const *U, int len, int kind,
     for (int i = 0; i < len; ++i)
       t[i] = a * (U[5][i] + U[4][i]) + b * (U[6][i] + U[3][i]) + cc * (U[7][i] + U[2][i]) + d * (U[8][i] + U[1][i]);
     break;
+  case 5:
+    for (int i = 0; i < len; ++i)
+      t[i] = a * ((U[5][i] + U[3][i]) - 2.0 * U[4][i]) + b * ((U[6][i] + U[2][i]) - 2.0 * U[4][i]);
+    break;
   default:
     for (int i = 0; i < len; ++i)
       t[i] = a * (U[4][i] + U[3][i]) + b * (U[5][i] + U[2][i]) + cc * (U[6][i] + U[1][i]) + d * (U[7][i] + U[0][i]);
@@ -468,9 +485,10 @@ static void rhs_run(double *t, const double *const *U, int len, int kind,
 int orc_apply(const orc_ctx *c, double *t, const double *u, int op, int dir,
               int kind, int nx, int ny, int nz)
 {
-  if (op < 0 || op >= ORC_NOPS || dir < 0 || dir > 2 || kind < 0 || kind > 4) return -1;
+  if (op < 0 || op >= ORC_NOPS_ALL || dir < 0 || dir > 2 || kind < 0 || kind > 5) return -1;
   double a, b, cc = 0.0, d = 0.0;
   if (kind == 0) { a = c->afi[dir]; b = c->bfi[dir]; }
+  else if (kind == 5) { a = c->asi[dir]; b = c->bsi[dir]; }
   else if (kind <= 2) { a = c->aci[dir]; b = c->bci[dir]; }
   else { a = c->aici[dir]; b = c->bici[dir]; cc = c->cici[dir]; d = c->dici[dir]; }
   int n = dir == 0 ? nx : dir == 1 ? ny : nz;

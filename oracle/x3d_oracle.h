@@ -27,7 +27,11 @@ enum {
   ORC_OP_DERXPV, ORC_OP_DERYPV, ORC_OP_DERZPV,
   ORC_OP_INTXVP, ORC_OP_INTYVP, ORC_OP_INTZVP,
   ORC_OP_INTXPV, ORC_OP_INTYPV, ORC_OP_INTZPV,
-  ORC_NOPS
+  ORC_NOPS,
+  /* second derivatives (derxx/deryy/derzz, declared and never implemented by
+   * the reference, src/x3d_derive.f90:54-59): same numbering as the library */
+  ORC_OP_DERXX = ORC_NOPS, ORC_OP_DERYY, ORC_OP_DERZZ,
+  ORC_NOPS_ALL
 };
 
 typedef struct {
@@ -43,7 +47,8 @@ typedef struct {
   double alfai[3], afi[3], bfi[3];                       /* 6th-order compact D   */
   double alcai[3], aci[3], bci[3];                       /* staggered Ds          */
   double ailcai[3], aici[3], bici[3], cici[3], dici[3];  /* mid-point interp. I   */
-  orc_op ops[ORC_NOPS];
+  double alsai[3], asi[3], bsi[3];                       /* compact 2nd deriv.    */
+  orc_op ops[ORC_NOPS_ALL];
   /* Poisson tables (src/poisson.f90:421-479, 483-767) */
   double *ax, *bx, *ay, *by, *az, *bz;
   double *xk2, *yk2, *zk2, *exs, *eys, *ezs;  /* real parts (imag parts equal) */

@@ -141,7 +141,13 @@ program ref_calls
   call derz (tg3,td3,x3d_op_derz ,zsize(1),zsize(2),zsize(3))
   call dump(tg3)
 
-  ! 12-21: one step of the driver's loop
+  ! 12: derzz through the procedure pointer (declared by the reference at
+  ! src/x3d_derive.f90:54-59 with the DERIVATIVE_Z interface, never associated
+  ! there)
+  call derzz (tg3,td3,x3d_op_derzz,zsize(1),zsize(2),zsize(3))
+  call dump(tg3)
+
+  ! 13-22: one step of the driver's loop
   itr = 1
   ! ref src/xcompact3d.f90:83
   call calculate_transeq_rhs(dux1,duy1,duz1,ux1,uy1,uz1)
@@ -155,7 +161,7 @@ program ref_calls
   call dump(dux1(:,:,:,1)); call dump(duy1(:,:,:,1)); call dump(duz1(:,:,:,1))
   call dump(pp3(:,:,:,1)); call dump(px1); call dump(py1); call dump(pz1)
 
-  ! 22-25: solve_poisson's three calls on the corrected velocity
+  ! 23-26: solve_poisson's three calls on the corrected velocity
   nlock = 1
   ! ref src/navier.f90:45
   CALL divergence(pp3(:,:,:,1),ux1,uy1,uz1,nlock)

@@ -69,7 +69,7 @@ def test_reference_call_sites_run(torch_cuda, tmp_path):
                        timeout=120, cwd=tmp_path)
     print(r.stdout[-1500:], r.stderr[-1500:])
     assert r.returncode == 0 and "ref_calls done" in r.stdout
-    out = _read(tmp_path / "ref_calls.bin", (nz, ny, nx), 26)
+    out = _read(tmp_path / "ref_calls.bin", (nz, ny, nx), 27)
 
     orc = Oracle(nx, ny, nz, 1.0, 1.0, 1.0)
     ux, uy, uz = orc.fields(0)
@@ -86,8 +86,9 @@ def test_reference_call_sites_run(torch_cuda, tmp_path):
         orc.thomas12("derx", uy),
         orc.thomas("dery", uz),
         orc.apply("derz", ux),
+        orc.apply("derzz", ux),
     ]
-    for i, (a, b) in enumerate(zip(out[:11], expect)):
+    for i, (a, b) in enumerate(zip(out[:12], expect)):
         assert _rel(a, b) < RTOL, (i + 1, _rel(a, b))
     # test_du's printed RMS error (src/mom.f90:59-243): the derivative is held
     # to 1e-12 of its max, and |rms(a - f) - rms(b - f)| <= max|a - b|
@@ -100,13 +101,13 @@ def test_reference_call_sites_run(torch_cuda, tmp_path):
     # and its gradient are rounding noise, held in absolute terms
     ru, rdu, rpp, rg = orc.step(ux, uy, uz, gdt=0.0)
     umax = max(np.abs(a).max() for a in ru)
-    for i, (a, b) in enumerate(zip(out[11:14] + out[14:17], ru + rdu)):
+    for i, (a, b) in enumerate(zip(out[12:15] + out[15:18], ru + rdu)):
         assert _rel(a, b) < RTOL, ("step", i, _rel(a, b))
-    for i, (a, b) in enumerate(zip(out[17:21], [rpp] + rg)):
+    for i, (a, b) in enumerate(zip(out[18:22], [rpp] + rg)):
         assert np.abs(a - b).max() < RTOL * umax, ("pressure", i, np.abs(a - b).max())
     # divergence, poisson and gradp of the corrected field: noise as well
     div = d(*ru)
-    assert np.abs(out[21] - div).max() < RTOL * umax
+    assert np.abs(out[22] - div).max() < RTOL * umax
     pp, g = orc.solve_poisson(*ru)
-    for i, (a, b) in enumerate(zip(out[22:26], [pp] + g)):
+    for i, (a, b) in enumerate(zip(out[23:27], [pp] + g)):
         assert np.abs(a - b).max() < RTOL * umax, ("poisson", i, np.abs(a - b).max())

@@ -95,6 +95,28 @@ def test_operator_parity(grid, name):
     assert relerr(host(t), ref) < RTOL
 
 
+@pytest.mark.parametrize("name", ["derxx", "deryy", "derzz"])
+def test_second_derivative_parity(grid, name):
+    """derxx / deryy / derzz (the compact sixth-order scheme; the oracle's is
+    pinned by its Fourier symbol): tables bitwise, FP64 at the 1e-12 bar,
+    FP32 within 2e-6 of the oracle on the same FP32-rounded input (the
+    second derivative's condition is that of derx: alfa = 2/11)."""
+    import torch
+    ctx, orc = grid
+    for a, b in zip(ctx.operator(name)[:4], orc.op(name)[:4]):
+        np.testing.assert_array_equal(a, b)
+    u = rand(orc, 21)
+    ref = orc.apply(name, u)
+    t = ctx.field(fill=np.nan)
+    ctx.apply(name, t, dev(u))
+    assert relerr(host(t), ref) < RTOL
+    u32 = u.astype(np.float32)
+    ref32 = orc.apply(name, u32.astype(np.float64))
+    t32 = ctx.field32(fill=np.nan)
+    ctx.apply_f32(name, t32, torch.from_numpy(u32).cuda())
+    assert relerr(host(t32).astype(np.float64), ref32) < 2e-6
+
+
 @pytest.mark.parametrize("axis", ["x", "y", "z"])
 @pytest.mark.parametrize("kind", ["der", "int"])
 def test_thomas_parity(grid, axis, kind):

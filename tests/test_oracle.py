@@ -152,6 +152,44 @@ def test_plain_sweeps_solve_the_factored_tridiagonal(axis):
     assert rel(xl @ A.T, rl) < 1e-13
 
 
+def sym_dd(w, h):
+    """Fourier symbol of the sixth-order compact second derivative (Lele
+    1992, alfa = 2/11, a = 12/11, b = 3/11): the factor it multiplies
+    e^{i w j} by."""
+    al, a, b = 2 / 11, 12 / 11, 3 / 11
+    return -(2 * a * (1 - np.cos(w)) + 0.5 * b * (1 - np.cos(2 * w))) / (h * h * (1 + 2 * al * np.cos(w)))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_second_derivative_symbol_and_order(axis):
+    """derxx / deryy / derzz: the reference declares them and implements no
+    scheme (src/x3d_derive.f90:54-59), so the oracle's is pinned by the
+    mathematics only.  Single modes give back their closed-form symbol (a
+    wrong sign, index or coefficient shows), the symbol matches -w^2/h^2 with
+    an error of order w^8 (sixth order), and random fields match the FFT of
+    the symbol."""
+    dims = [8, 8, 8]
+    dims[axis] = 48
+    L = [1.0, 2.0, 3.0]
+    o = Oracle(*dims, *L)
+    n, h = dims[axis], L[axis] / dims[axis]
+    name = "der" + "xyz"[axis] * 2
+    x = np.arange(n)
+    shape = [1, 1, 1]
+    shape[2 - axis] = n
+    for k in (1, 3, 7, 24):
+        w = 2 * np.pi * k / n
+        u = np.broadcast_to(np.cos(w * x + 0.3).reshape(shape), o.shape).copy()
+        got = o.apply(name, u)
+        assert rel(got, sym_dd(w, h) * u) < 1e-12, k
+    e = [abs(sym_dd(w, 1.0) + w * w) for w in (0.2, 0.1)]
+    assert 230 < e[0] / e[1] < 280  # 2^8
+    r = np.random.default_rng(axis).standard_normal(o.shape)
+    U = np.fft.fft(r, axis=2 - axis)
+    m = sym_dd(2 * np.pi * np.fft.fftfreq(n), h).reshape(shape)
+    assert rel(o.apply(name, r), np.real(np.fft.ifft(U * m, axis=2 - axis))) < 1e-12
+
+
 def test_operator_tables_equal_between_pairs(orc):
     # derX and derXp, intXvp and intXpv, derXvp and derXpv share tables (periodic)
     for a, b in (("derx", "derxp"), ("intxvp", "intxpv"), ("derxvp", "derxpv"), ("derz", "derzp")):

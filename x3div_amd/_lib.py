@@ -43,6 +43,9 @@ SIGNATURES = {
     "x3d_derx": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
     "x3d_dery": (_i, [_vp, _dp, _dp, _i, _dp, _i, _i, _i]),
     "x3d_derz": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
+    "x3d_derxx": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
+    "x3d_deryy": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
+    "x3d_derzz": (_i, [_vp, _dp, _dp, _i, _i, _i, _i]),
     "x3d_derxvp": (_i, [_vp, _dp, _dp, _i, _i, _i, _i, _i]),
     "x3d_interxvp": (_i, [_vp, _dp, _dp, _i, _i, _i, _i, _i]),
     "x3d_derxpv": (_i, [_vp, _dp, _dp, _i, _i, _i, _i, _i]),

@@ -21,7 +21,9 @@ from ._lib import check
 OPS = ["derx", "derxp", "dery", "deryp", "derz", "derzp",
        "derxvp", "deryvp", "derzvp", "derxpv", "derypv", "derzpv",
        "intxvp", "intyvp", "intzvp", "intxpv", "intypv", "intzpv"]
-OP = {name: i for i, name in enumerate(OPS)}
+# second derivatives (derxx/deryy/derzz), after the reference's 18 instances
+OPS2 = ["derxx", "deryy", "derzz"]
+OP = {name: i for i, name in enumerate(OPS + OPS2)}
 
 FIELD_REFERENCE_VEL, FIELD_TG_NOISE, FIELD_FOURIER8 = 0, 1, 2
 
@@ -113,7 +115,7 @@ class X3D:
     def operator(self, op):
         """(f, s, w, periodic, alfa, npaire) of one x3doperator1d instance."""
         o = _op_id(op)
-        n = {0: self.nx, 1: self.ny, 2: self.nz}[_axis(OPS[o])]
+        n = {0: self.nx, 1: self.ny, 2: self.nz}[_axis((OPS + OPS2)[o])]
         arrs = [np.zeros(n) for _ in range(4)]
         alfa = C.c_double(); nn = C.c_int(); npaire = C.c_int()
         hp = C.POINTER(C.c_double)
@@ -175,6 +177,16 @@ class X3D:
     def derz(self, tz, uz, op="derz"):
         check("x3d_derz", self.lib.x3d_derz(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz))
 
+    # ---- second derivatives (derxx/deryy/derzz, src/x3d_derive.f90:54-59) ----------
+    def derxx(self, tx, ux, op="derxx"):
+        check("x3d_derxx", self.lib.x3d_derxx(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.ny, self.nz))
+
+    def deryy(self, ty, uy, op="deryy"):
+        check("x3d_deryy", self.lib.x3d_deryy(self._h, self._f(ty), self._f(uy), _op_id(op), self.nx, self.ny, self.nz))
+
+    def derzz(self, tz, uz, op="derzz"):
+        check("x3d_derzz", self.lib.x3d_derzz(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz))
+
     # ---- staggered (src/x3d_staggered.f90) -----------------------------------------
     def derxvp(self, tx, ux, op="derxvp"):
         check("x3d_derxvp", self.lib.x3d_derxvp(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.nx, self.ny, self.nz))
@@ -220,6 +232,7 @@ class X3D:
             "derzvp": self.derzvp, "derxpv": self.derxpv, "derypv": self.derypv, "derzpv": self.derzpv,
             "intxvp": self.interxvp, "intyvp": self.interyvp, "intzvp": self.interzvp,
             "intxpv": self.interxpv, "intypv": self.interypv, "intzpv": self.interzpv,
+            "derxx": self.derxx, "deryy": self.deryy, "derzz": self.derzz,
         }[name]
         routine(out, inp, name)
 

@@ -85,6 +85,12 @@ static void scheme(Scheme& s, double h)
   s.bici = 1.0 / 256.0 * (126.0 * s.ailcai - 25.0);
   s.cici = 1.0 / 256.0 * (-10.0 * s.ailcai + 3.0);
   s.dici = 0.0;
+  // second derivative, isecondder = 4 (src/parameters.f90:107): the
+  // sixth-order tridiagonal scheme of Lele (1992), alfa = 2/11, a = 12/11,
+  // b = 3/11 on the stencils (u+1 - 2u + u-1)/h^2 and (u+2 - 2u + u-2)/(4h^2)
+  s.alsai = 2.0 / 11.0;
+  s.asi = (12.0 / 11.0) / (h * h);
+  s.bsi = (3.0 / 44.0) / (h * h);
 }
 
 // The six operators of one direction (x3d_operator_1d_init, :76-189, periodic).
@@ -104,12 +110,15 @@ static void direction_ops(x3d_ctx* c, int d, int n)
   prepare(b, cd, f, s, w);
   op_init(c->op[X3D_OP_INTXVP + d], f, s, w, sc.ailcai, 1);    // cifxp6 == cifx6 here
   op_init(c->op[X3D_OP_INTXPV + d], f, s, w, sc.ailcai, 1);
+  periodic_rows(sc.alsai, n, f, cd, b);
+  prepare(b, cd, f, s, w);
+  op_init(c->op[X3D_OP_DERXX + d], f, s, w, sc.alsai, 0);
 }
 
 static int upload_tables(x3d_ctx* c)
 {
   size_t len = 0;
-  for (int o = 0; o < X3D_NOPS; ++o) { c->off[o] = len; len += 4 * (size_t)c->op[o].n; }
+  for (int o = 0; o < X3D_NOPS_ALL; ++o) { c->off[o] = len; len += 4 * (size_t)c->op[o].n; }
   if (!c->tab64 || c->tab_len != len) {
     if (c->tab64) (void)hipFree(c->tab64);
     if (c->tab32) (void)hipFree(c->tab32);
@@ -120,7 +129,7 @@ static int upload_tables(x3d_ctx* c)
   }
   std::vector<double> h(len);
   std::vector<float> hf(len);
-  for (int o = 0; o < X3D_NOPS; ++o) {
+  for (int o = 0; o < X3D_NOPS_ALL; ++o) {
     const OpHost& op = c->op[o];
     const size_t n = op.n;
     std::memcpy(&h[c->off[o]], op.f.data(), n * sizeof(double));
@@ -426,6 +435,7 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   direction_ops(c, 0, nx);
   direction_ops(c, 1, ny);
   direction_ops(c, 2, nz);
+  if (const char* xr = std::getenv("X3D_XCD_REMAP")) c->xcd_remap = std::atoi(xr) != 0;
   // tiling variant of the fused y/z passes (X3D_FUSED_TILING=0|1, default 0)
   const char* tv = std::getenv("X3D_FUSED_TILING");
   const int variant = tv ? std::atoi(tv) : 0;
@@ -532,7 +542,7 @@ int x3d_memcpy_d2h(x3d_ctx* c, void* dst, const void* src, size_t nbytes)
 int x3d_operator_get(const x3d_ctx* c, int op, double* f, double* s, double* w, double* p,
                      double* alfa, int* n, int* npaire)
 {
-  if (!c || op < 0 || op >= X3D_NOPS) return X3D_ERR_ARG;
+  if (!c || op < 0 || op >= X3D_NOPS_ALL) return X3D_ERR_ARG;
   const OpHost& o = c->op[op];
   const size_t nb = sizeof(double) * o.n;
   if (f) std::memcpy(f, o.f.data(), nb);
@@ -548,7 +558,7 @@ int x3d_operator_get(const x3d_ctx* c, int op, double* f, double* s, double* w, 
 int x3d_operator_set(x3d_ctx* c, int op, const double* f, const double* s, const double* w,
                      const double* p, double alfa)
 {
-  if (!c || op < 0 || op >= X3D_NOPS || !f || !s || !w || !p) return X3D_ERR_ARG;
+  if (!c || op < 0 || op >= X3D_NOPS_ALL || !f || !s || !w || !p) return X3D_ERR_ARG;
   OpHost& o = c->op[op];
   const int n = o.n;
   const size_t nb = (size_t)n * sizeof(double);
@@ -682,6 +692,24 @@ int x3d_derz(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int
 {
   int rc = check3(c, nx, ny, nz);
   return rc ? rc : launch_line(c, 2, K_D, op, u, t, true);
+}
+
+// derxx / deryy / derzz (declared at src/x3d_derive.f90:54-59): the compact
+// second derivative, periodic, through the same line kernels.
+int x3d_derxx(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 0, K_DD, op, u, t, true);
+}
+int x3d_deryy(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 1, K_DD, op, u, t, true);
+}
+int x3d_derzz(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int nz)
+{
+  int rc = check3(c, nx, ny, nz);
+  return rc ? rc : launch_line(c, 2, K_DD, op, u, t, true);
 }
 
 // ---- staggered (periodic: nXm = nX) --------------------------------------
@@ -937,10 +965,11 @@ int x3d_derz_f32(x3d_ctx* c, float* t, const float* u, int op, int nx, int ny, i
 
 int x3d_apply_f32(x3d_ctx* c, int op, float* out, const float* in)
 {
-  if (!c || op < 0 || op >= X3D_NOPS) return X3D_ERR_ARG;
-  // operator ids: 0-5 D (x, x, y, y, z, z); then Dvp, Dpv, Ivp, Ipv, each x, y, z
+  if (!c || op < 0 || op >= X3D_NOPS_ALL) return X3D_ERR_ARG;
+  // operator ids: 0-5 D (x, x, y, y, z, z); then Dvp, Dpv, Ivp, Ipv, each
+  // x, y, z; then the second derivatives x, y, z
   const int dir = op < 6 ? op / 2 : (op - 6) % 3;
-  static const int kinds[4] = {K_DVP, K_DPV, K_IVP, K_IPV};
+  static const int kinds[5] = {K_DVP, K_DPV, K_IVP, K_IPV, K_DD};
   const int kind = op < 6 ? K_D : kinds[(op - 6) / 3];
   return launch_line_f32(c, dir, kind, op, in, out);
 }

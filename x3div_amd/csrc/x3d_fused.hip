@@ -348,13 +348,13 @@ __device__ __forceinline__ int grid_of(const int* __restrict__ block0, int ngrid
 }
 
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) void k_transeq(TranseqArgs<T> A) { transeq_body<T, M, DIR>(A, blockIdx.x); }
+__global__ __launch_bounds__(512) void k_transeq(TranseqArgs<T> A) { transeq_body<T, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
 // y passes: at most 128 VGPRs, so that two work-groups share a compute unit
 // (measured faster at 512^3 in both precisions despite a few spills)
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_div(PassArgs<T> A) { div_body<T, M, DIR>(A, blockIdx.x); }
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_div(PassArgs<T> A) { div_body<T, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
 template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_grad(PassArgs<T> A) { grad_body<T, M, DIR>(A, blockIdx.x); }
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_grad(PassArgs<T> A) { grad_body<T, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
 
 template <typename T, int M, int DIR>
 __global__ __launch_bounds__(512) void k_transeq_batch(const TranseqArgs<T>* __restrict__ list,

@@ -19,7 +19,8 @@ enum Kind : int {
   K_DPV = 2,  // staggered derivative, pressure -> velocity grid
   K_IVP = 3,  // mid-point interpolation, velocity -> pressure grid
   K_IPV = 4,  // mid-point interpolation, pressure -> velocity grid
-  K_NONE = 5  // no stencil: the Thomas solve alone (in place)
+  K_NONE = 5,  // no stencil: the Thomas solve alone (in place)
+  K_DD = 6     // collocated second derivative (derxx)
 };
 
 // Scheme constants of one direction (module x3d_operator_?_data).
@@ -27,6 +28,7 @@ struct Scheme {
   double alfai, afi, bfi;                    // compact first derivative
   double alcai, aci, bci;                    // staggered derivative
   double ailcai, aici, bici, cici, dici;     // interpolation (ipinter = 3)
+  double alsai, asi, bsi;                    // compact second derivative (isecondder = 4)
 };
 
 // Host copy of one x3doperator1d instance.
@@ -70,11 +72,11 @@ struct x3d_ctx {
   bool f64_lines = true;  // FP64 fields below 4 GiB (32-bit offsets of the line kernels)
   hipStream_t stream = nullptr;
   x3d::Scheme sc[3];
-  x3d::OpHost op[X3D_NOPS];
+  x3d::OpHost op[X3D_NOPS_ALL];
   // device operator tables: for op o, f,s,w,p at tab64 + off[o] + {0,1,2,3}*n
   double* tab64 = nullptr;
   float* tab32 = nullptr;
-  size_t off[X3D_NOPS] = {};
+  size_t off[X3D_NOPS_ALL] = {};
   size_t tab_len = 0;
   x3d::LinePlan plan[3];   // single-operator kernels
   x3d::LinePlan fplan[3];  // fused step passes (at most 512 threads per work-group)
@@ -91,6 +93,7 @@ struct x3d_ctx {
   bool ysplit = false;              // and x-only hipFFT plans + y stage kernels (ny a power of two <= 512)
   void* ytw = nullptr;              // e^{-2 pi i t / ny}, t < ny (double2)
   int ncu = 0;                      // compute units of the context's device
+  int xcd_remap = 1;                // line blocks regrouped per XCD (X3D_XCD_REMAP=0 turns it off)
   int spl_wg[2][2] = {{0, 0}, {0, 0}};  // resident work-groups/CU of the Poisson line kernels [FP64/FP32][z/y]
   std::vector<double> ax, bx, ay, by, az, bz;
   std::vector<double> xk2, yk2, zk2, exs, eys, ezs;

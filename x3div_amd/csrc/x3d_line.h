@@ -31,6 +31,7 @@ template <> struct Halo<K_DVP> { static constexpr int L = 1, R = 2; };
 template <> struct Halo<K_DPV> { static constexpr int L = 2, R = 1; };
 template <> struct Halo<K_IVP> { static constexpr int L = 3, R = 4; };
 template <> struct Halo<K_IPV> { static constexpr int L = 4, R = 3; };
+template <> struct Halo<K_DD> { static constexpr int L = 2, R = 2; };
 
 template <typename T>
 struct Coef { T a, b, c, d; };
@@ -41,6 +42,8 @@ struct Coef { T a, b, c, d; };
 // DPV  derxpv       src/x3d_staggered.f90:246-249
 // IVP  interxvp     src/x3d_staggered.f90:143-148
 // IPV  interxpv     src/x3d_staggered.f90:326-331
+// DD   derxx        (declared only, src/x3d_derive.f90:54-59): sixth-order
+//                   compact second derivative, a (u+1 - 2u + u-1) + b (u+2 - 2u + u-2)
 // (the wrapped boundary rows are the same formulas with indices mod n)
 template <int KIND, typename T>
 __device__ __forceinline__ T rhs_at(const T* win, int q, const Coef<T>& k)
@@ -57,6 +60,8 @@ __device__ __forceinline__ T rhs_at(const T* win, int q, const Coef<T>& k)
   else if constexpr (KIND == K_IPV)
     return k.a * (win[q] + win[q - 1]) + k.b * (win[q + 1] + win[q - 2]) +
            k.c * (win[q + 2] + win[q - 3]) + k.d * (win[q + 3] + win[q - 4]);
+  else if constexpr (KIND == K_DD)
+    return k.a * ((win[q + 1] + win[q - 1]) - (T)2 * win[q]) + k.b * ((win[q + 2] + win[q - 2]) - (T)2 * win[q]);
   else
     return win[q];
 }
@@ -71,7 +76,20 @@ struct Geom {
 
 struct Tiling {
   int G, Wc, Wl, LW, line_fast, fix;
+  int xcd;  // single-grid launches: line blocks regrouped per XCD (xcd_block)
 };
+
+// Work-group b of a launch of nb runs on XCD b % 8: the dispatcher deals
+// work-groups to the 8 XCDs in turn.  This gives XCD x the contiguous range
+// of line blocks [x nb / 8, (x + 1) nb / 8) instead, so that the blocks that
+// share rows (neighbouring x ranges of the same y/z rows, each row in its own
+// 2 MiB page at 512^3) run on one XCD and share its L2 and address
+// translation caches.  A bijection of [0, nb) for any nb.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
 
 // Constants of the circulant solver for one alfa and one tiling.
 template <typename T>

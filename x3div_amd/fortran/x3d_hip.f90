@@ -95,6 +95,21 @@ module x3d_hip
       type(c_ptr), value :: ctx, t, u
       integer(c_int), value :: op, nx, ny, nz
     end function
+    integer(c_int) function x3d_derxx(ctx, t, u, op, nx, ny, nz) bind(C, name="x3d_derxx")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, t, u
+      integer(c_int), value :: op, nx, ny, nz
+    end function
+    integer(c_int) function x3d_deryy(ctx, t, u, op, nx, ny, nz) bind(C, name="x3d_deryy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, t, u
+      integer(c_int), value :: op, nx, ny, nz
+    end function
+    integer(c_int) function x3d_derzz(ctx, t, u, op, nx, ny, nz) bind(C, name="x3d_derzz")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, t, u
+      integer(c_int), value :: op, nx, ny, nz
+    end function
     integer(c_int) function x3d_deryvp(ctx, t, u, op, pp, nx, ny, nym, nz) bind(C, name="x3d_deryvp")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, t, u, pp

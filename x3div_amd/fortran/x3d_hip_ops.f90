@@ -35,6 +35,10 @@ module x3d_operator_1d
   type(x3doperator1d), save, public, target :: x3d_op_derxpv, x3d_op_derypv, x3d_op_derzpv
   type(x3doperator1d), save, public, target :: x3d_op_intxvp, x3d_op_intyvp, x3d_op_intzvp
   type(x3doperator1d), save, public, target :: x3d_op_intxpv, x3d_op_intypv, x3d_op_intzpv
+  ! second derivatives for derxx / deryy / derzz: this build's instances (the
+  ! reference declares the procedure pointers, src/x3d_derive.f90:54-59, and
+  ! no operator for them)
+  type(x3doperator1d), save, public, target :: x3d_op_derxx, x3d_op_deryy, x3d_op_derzz
 
   public :: x3d_operator_1d_init, x3d_operator_1d_finalize, x3d_op_by_id
 
@@ -53,6 +57,7 @@ contains
     call fill(x3d_op_derxpv, 9); call fill(x3d_op_derypv, 10); call fill(x3d_op_derzpv, 11)
     call fill(x3d_op_intxvp, 12); call fill(x3d_op_intyvp, 13); call fill(x3d_op_intzvp, 14)
     call fill(x3d_op_intxpv, 15); call fill(x3d_op_intypv, 16); call fill(x3d_op_intzpv, 17)
+    call fill(x3d_op_derxx, 18); call fill(x3d_op_deryy, 19); call fill(x3d_op_derzz, 20)
 #ifdef DEBUG
     if (nrank == 0) write (*, *) '# x3d_operator_1d_init done'
 #endif
@@ -86,6 +91,7 @@ contains
     call x3d_op_derxpv%finalize(); call x3d_op_derypv%finalize(); call x3d_op_derzpv%finalize()
     call x3d_op_intxvp%finalize(); call x3d_op_intyvp%finalize(); call x3d_op_intzvp%finalize()
     call x3d_op_intxpv%finalize(); call x3d_op_intypv%finalize(); call x3d_op_intzpv%finalize()
+    call x3d_op_derxx%finalize(); call x3d_op_deryy%finalize(); call x3d_op_derzz%finalize()
   end subroutine x3d_operator_1d_finalize
 
   !> x3doperator1d%init (src/x3d_operator_1d.f90:234-266): point f, s, w at
@@ -143,7 +149,7 @@ contains
     integer :: k
     a = c_loc(x3dop%n)
     id = -1
-    do k = 0, 17
+    do k = 0, 20
       if (c_associated(a, c_loc(instance_n(k)))) id = k
     end do
   end function instance_id
@@ -157,7 +163,7 @@ contains
     type(x3doperator1d), pointer :: o
     integer :: k
     id = -1
-    do k = 0, 17
+    do k = 0, 20
       o => instance(k)
       if (.not. associated(o%f) .or. .not. allocated(o%periodic)) cycle
       if (size(o%f) /= size(ff)) cycle
@@ -199,7 +205,10 @@ contains
     case (14); o => x3d_op_intzvp
     case (15); o => x3d_op_intxpv
     case (16); o => x3d_op_intypv
-    case default; o => x3d_op_intzpv
+    case (17); o => x3d_op_intzpv
+    case (18); o => x3d_op_derxx
+    case (19); o => x3d_op_deryy
+    case default; o => x3d_op_derzz
     end select
   end function instance
 
@@ -329,8 +338,10 @@ module x3d_derive
   use x3d_operator_1d, only : x3doperator1d
   implicit none
   private
-  public :: DERIVATIVE_X, DERIVATIVE_Y, DERIVATIVE_Z, derx, dery, derz
+  public :: DERIVATIVE_X, DERIVATIVE_Y, DERIVATIVE_YY, DERIVATIVE_Z, derx, dery, derz
+  public :: derxx, deryy, derzz
   public :: x3d_derive_init, x3d_derive_finalize, derx_00, dery_00, derz_00
+  public :: derxx_00, deryy_00, derzz_00
 
   abstract interface
     subroutine DERIVATIVE_X(t, u, x3dop, nx, ny, nz)
@@ -348,6 +359,13 @@ module x3d_derive
       real(mytype), intent(in), dimension(ny) :: pp
       type(x3doperator1d), intent(in) :: x3dop
     end subroutine DERIVATIVE_Y
+    subroutine DERIVATIVE_YY(t, u, x3dop, nx, ny, nz)
+      import :: mytype, x3doperator1d
+      integer, intent(in) :: nx, ny, nz
+      real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+      real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+      type(x3doperator1d), intent(in) :: x3dop
+    end subroutine DERIVATIVE_YY
     subroutine DERIVATIVE_Z(t, u, x3dop, nx, ny, nz)
       import :: mytype, x3doperator1d
       integer, intent(in) :: nx, ny, nz
@@ -357,9 +375,10 @@ module x3d_derive
     end subroutine DERIVATIVE_Z
   end interface
 
-  procedure(DERIVATIVE_X), pointer :: derx => null()
+  procedure(DERIVATIVE_X), pointer :: derx => null(), derxx => null()
   procedure(DERIVATIVE_Y), pointer :: dery => null()
-  procedure(DERIVATIVE_Z), pointer :: derz => null()
+  procedure(DERIVATIVE_YY), pointer :: deryy => null()
+  procedure(DERIVATIVE_Z), pointer :: derz => null(), derzz => null()
 
 contains
 
@@ -368,11 +387,41 @@ contains
     derx => derx_00
     dery => dery_00
     derz => derz_00
+    ! second derivatives: the reference leaves these pointers unassociated
+    derxx => derxx_00
+    deryy => deryy_00
+    derzz => derzz_00
   end subroutine x3d_derive_init
 
   subroutine x3d_derive_finalize()
-    nullify (derx, dery, derz)
+    nullify (derx, dery, derz, derxx, deryy, derzz)
   end subroutine x3d_derive_finalize
+
+  !> derxx / deryy / derzz, periodic: the sixth-order compact second derivative
+  !> (x3d_op_derxx / deryy / derzz) on the library's line kernels.
+  subroutine derxx_00(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call x3d_check('derxx', x3d_derxx(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), nx, ny, nz))
+  end subroutine derxx_00
+
+  subroutine deryy_00(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call x3d_check('deryy', x3d_deryy(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), nx, ny, nz))
+  end subroutine deryy_00
+
+  subroutine derzz_00(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call x3d_check('derzz', x3d_derzz(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), nx, ny, nz))
+  end subroutine derzz_00
 
   subroutine derx_00(t, u, x3dop, nx, ny, nz)
     integer, intent(in) :: nx, ny, nz
