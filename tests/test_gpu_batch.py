@@ -83,3 +83,30 @@ def test_batch_autotune_keeps_results(torch_cuda):
         for name, r in zip(("ux", "uy", "uz"), u):
             assert relerr(b.get(i, name), r) < 1e-12, (i, name)
     b.close()
+
+
+def test_benchmark_batch_subset_tuned_and_captured(torch_cuda):
+    """The bench's own batch: the first 24 of its 96 seeded grids (all six
+    sizes 32..160 appear), autotuned, then stepped through the captured graph,
+    each grid against the oracle's step."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from bench import BATCH_SIZES, batch_dims
+    from x3div_amd import X3DBatch
+    dims = batch_dims()[:24]
+    assert set(BATCH_SIZES) <= {v for d in dims for v in d}, dims
+    print(f"\n[bench batch subset: {len(dims)} grids]", flush=True)
+    b = X3DBatch(dims, precision=8)
+    print(b.autotune(reps=1))
+    b.init_fields(kind=2, seed=SEED)
+    b.graph_capture(gdt=1e-3)
+    b.graph_launch()
+    ref = oracle_steps(dims, 1e-3)
+    for i, (u, du, pp, g) in enumerate(ref):
+        for name, r in zip(("ux", "uy", "uz", "dux", "duy", "duz"), u + du):
+            assert relerr(b.get(i, name), r) < 1e-12, (i, dims[i], name)
+        assert relerr(b.get(i, "pp3"), pp) < 1e-11, (i, dims[i])
+        for name, r in zip(("px", "py", "pz"), g):
+            assert relerr(b.get(i, name), r) < 1e-11, (i, dims[i], name)
+    b.close()
