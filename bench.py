@@ -40,9 +40,10 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256, help="grid of the all-threads CPU-baseline steps")
     ap.add_argument("--cpu-n1", type=int, default=128, help="grid of the one-thread CPU-baseline steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--forms", default="fp32,batch,large",
+    ap.add_argument("--forms", default="fp32,bfp32,batch,large",
                     help="comma list of extra forms to measure beside plain FP64: fp32 (precision "
-                         "form at n^3), batch (96 uneven grids, FP64 and FP32), large (FP32 at "
+                         "form at n^3), bfp32 (block-floating-point fields, FP64 arithmetic, at "
+                         "n^3), batch (96 uneven grids, FP64 and FP32), large (FP32 at "
                          "1024x768x768 beside FP64 at 768^3); empty for none")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch a plain device copy of one field (counter calibration) "
@@ -221,7 +222,7 @@ def measure_operators(ctx, src, dst, reps=10):
     return out
 
 
-def roofline_of(passes, n, gpu_ms_step, suffix, npts=None):
+def roofline_of(passes, n, gpu_ms_step, suffix, npts=None, stype=None):
     """The slowest line pass (the Poisson pass is several hipFFT kernels and is
     reported beside it) against the HBM peak, plus the whole step's fraction."""
     npts = npts or n ** 3
@@ -230,7 +231,7 @@ def roofline_of(passes, n, gpu_ms_step, suffix, npts=None):
     alg_bytes = int(passes[dom]["bytes_per_point"] * npts)
     achieved = alg_bytes / (kms / 1e3) / 1e9
     step_bytes = sum(p["bytes_per_point"] for p in passes.values()) * npts
-    kern = KERNEL_OF_PASS[dom].replace("double", "float") if suffix else KERNEL_OF_PASS[dom]
+    kern = KERNEL_OF_PASS[dom].replace("double", stype or ("float" if suffix else "double"))
     return {
         "bound": "hbm",
         "kernel": f"{dom} ({kern})",
@@ -311,6 +312,68 @@ def measure_fp32(ctx, u0, args, ms_step):
             "step_rel_err_vs_fp64": step_err,
             "operator_max_rel_err_vs_fp64_same_input": op_err,
             "operator_max_rel_err_vs_fp64_incl_input_rounding": op_err_in}
+
+
+def measure_bfp(ctx, u0, args, ms_step):
+    """The BFP32 form on the same grid: the fields' BFP32 words (4 B per
+    point, exponent per 16-point x-segment), FP64 stencils and solves.  Its
+    errors are taken against the plain form on the unrounded FP64 fields, so
+    they include the rounding of the input."""
+    import torch
+    from x3div_amd import OPS, OPS2
+    n = ctx.nx
+    q0 = []
+    for t in u0:
+        q = ctx.bfp_empty()
+        ctx.bfp_encode(q, t)
+        q0.append(q)
+    fb = [q.clone() for q in q0] + [ctx.bfp_empty() for _ in range(7)]
+    msb, gpu_msb, passesb = time_form(ctx, fb, ctx.step_bfp, ctx.step_timed_bfp, args)
+    del fb
+    # one step against the plain form's step on the unrounded fields
+    g64 = [t.clone() for t in u0] + [ctx.field() for _ in range(7)]
+    gb = [q.clone() for q in q0] + [ctx.bfp_empty() for _ in range(7)]
+    ctx.step(*g64, gdt=0.0)
+    ctx.step_bfp(*gb, gdt=0.0)
+    dec = ctx.field()
+    step_err = {}
+    for i, name in ((0, "ux"), (3, "dux"), (6, "pp3"), (7, "px")):
+        ctx.bfp_decode(dec, gb[i])
+        torch.cuda.synchronize()
+        step_err[name] = rel_errs(dec, g64[i])
+    del g64, gb
+    # every operator: error against FP64 on the unrounded field, and time
+    # (4 B read + 4 B written per point)
+    ref, outq = ctx.field(), ctx.bfp_empty()
+    op_err, op_ms = {}, {}
+    for name in OPS + OPS2:
+        ctx.apply(name, ref, u0[0])
+        ctx.apply_bfp(name, outq, q0[0])
+        ctx.bfp_decode(dec, outq)
+        torch.cuda.synchronize()
+        op_err[name] = rel_errs(dec, ref)["max"]
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ctx.stream)
+        for _ in range(args.kernel_reps):
+            ctx.apply_bfp(name, outq, q0[0])
+        e1.record(ctx.stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.kernel_reps
+        gbs = 8 * n ** 3 / (ms / 1e3) / 1e9
+        op_ms[name] = {"ms": round(ms, 4), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    del ref, outq, dec, q0
+    worst = max(op_err.values())
+    return {"value": round(n ** 3 / (msb / 1e3) / 1e6, 2), "ms_per_step": round(msb, 4),
+            "dtype": "bfp32 fields (31-bit mantissa, exponent per 16-point x-segment), f64 stencils, "
+                     "solves and Poisson",
+            "speedup_vs_fp64": round(ms_step / msb, 3),
+            "roofline": roofline_of(passesb, n, gpu_msb, "_bfp", stype="bfp32"),
+            "step_rel_err_vs_fp64_incl_input_rounding": step_err,
+            "operator_max_rel_err_vs_fp64_incl_input_rounding": op_err,
+            "operator_worst_rel_err": worst,
+            "operators_within_1e-6": bool(worst <= 1e-6),
+            "operators": op_ms}
 
 
 BATCH_SIZES = (32, 48, 64, 96, 128, 160)
@@ -423,6 +486,10 @@ def main():
     if "fp32" in extra:
         forms["fp32"] = measure_fp32(ctx, u0, args, ms_step)
         log(f"fp32 {n}^3: {forms['fp32']['ms_per_step']:.3f} ms/step")
+    if "bfp32" in extra and n % 16 == 0:
+        forms["bfp32"] = measure_bfp(ctx, u0, args, ms_step)
+        log(f"bfp32 {n}^3: {forms['bfp32']['ms_per_step']:.3f} ms/step, worst operator error "
+            f"{forms['bfp32']['operator_worst_rel_err']:.2e}")
     ctx.close()
     del f, u0
     torch.cuda.empty_cache()

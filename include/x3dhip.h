@@ -256,6 +256,34 @@ int x3d_step_timed_f32(x3d_ctx *ctx, double gdt, float *ux, float *uy, float *uz
                        float *dux, float *duy, float *duz, float *pp3,
                        float *px, float *py, float *pz, float *pass_ms);
 
+/* ---- BFP32 form: block-floating-point fields, FP64 line solves -------------
+ * A BFP32 field is an int32 array of the grid's shape (4 bytes per point, as
+ * FP32).  x is cut into 16-point segments (nx must be a multiple of 16; the
+ * field below 4 GiB); the values of a segment share the exponent e, the
+ * smallest with |v| < 2^e over the segment, clamped to [-126, 127].  Word k
+ * of a segment is (q << 1) | b with q = rint(v 2^(30-e)) clamped to
+ * [-2^30, 2^30 - 1] (so v = q 2^(e-30), within 2^-29 of the segment's largest
+ * value, 2^-30 but for values within half a unit of 2^e) and b = bit k of
+ * e + 128 for k < 8, 0 for k >= 8.  The operators and the step read and write
+ * BFP32 and compute in FP64, so they differ from the FP64 entry points on the
+ * decoded input by the rounding of their output; against the FP64 entry
+ * points on the unrounded field the rounding of the input is amplified as in
+ * the FP32 form, from a 64-128 times smaller start.  These entry points have
+ * no counterpart in the reference (real(mytype) fields, src/decomp_2d.f90);
+ * they give callers the FP32 form's traffic without FP32's rounding. */
+int x3d_bfp_encode(x3d_ctx *ctx, int32_t *q, const double *u);
+int x3d_bfp_decode(x3d_ctx *ctx, double *u, const int32_t *q);
+/* any operator along its own direction, ids as x3d_apply_f32; out != in */
+int x3d_apply_bfp(x3d_ctx *ctx, int op, int32_t *out, const int32_t *in);
+/* the step of x3d_step on BFP32 fields (Poisson in FP64 on the decoded
+ * right-hand side, the pressure stored back as BFP32) */
+int x3d_step_bfp(x3d_ctx *ctx, double gdt, int32_t *ux, int32_t *uy, int32_t *uz,
+                 int32_t *dux, int32_t *duy, int32_t *duz, int32_t *pp3,
+                 int32_t *px, int32_t *py, int32_t *pz);
+int x3d_step_timed_bfp(x3d_ctx *ctx, double gdt, int32_t *ux, int32_t *uy, int32_t *uz,
+                       int32_t *dux, int32_t *duy, int32_t *duz, int32_t *pp3,
+                       int32_t *px, int32_t *py, int32_t *pz, float *pass_ms);
+
 /* ---- captured step --------------------------------------------------------
  * Record one x3d_step (or x3d_step_f32) with these arrays into a HIP graph
  * held by the context (replacing any earlier one); x3d_graph_launch replays

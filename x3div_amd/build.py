@@ -17,7 +17,8 @@ LIB = os.path.join(HERE, "libx3dhip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-SOURCES = ["x3d_api.hip", "x3d_batch.hip", "x3d_compact.hip", "x3d_fused.hip", "x3d_mfma.hip", "x3d_poisson.hip"]
+SOURCES = ["x3d_api.hip", "x3d_batch.hip", "x3d_bfp.hip", "x3d_compact.hip", "x3d_fused.hip", "x3d_mfma.hip",
+           "x3d_poisson.hip"]
 
 CFLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -324,6 +324,43 @@ class X3D:
         return {self.lib.x3d_pass_name(i).decode(): (ms[i], self.lib.x3d_pass_bytes_per_point(i) / 2)
                 for i in range(X3D_NPASS)}
 
+    # ---- BFP32 form (block-floating-point int32 fields, FP64 line solves) ----------
+    def _bfp(self, t):
+        import torch
+        return _ptr(t, self.shape, torch.int32)
+
+    def bfp_empty(self):
+        """An uninitialised BFP32 field (int32 words) of the grid's shape."""
+        import torch
+        return torch.empty(self.shape, dtype=torch.int32, device="cuda")
+
+    def bfp_encode(self, q, u):
+        """q <- the BFP32 words of the FP64 field u (nx a multiple of 16)."""
+        check("x3d_bfp_encode", self.lib.x3d_bfp_encode(self._h, self._bfp(q), _ptr(u, self.shape)))
+
+    def bfp_decode(self, u, q):
+        """u <- the FP64 values of the BFP32 field q (exact)."""
+        check("x3d_bfp_decode", self.lib.x3d_bfp_decode(self._h, _ptr(u, self.shape), self._bfp(q)))
+
+    def apply_bfp(self, name: str, out, inp):
+        """Any operator, by instance name, along its own direction, on BFP32 fields (FP64 arithmetic)."""
+        check("x3d_apply_bfp", self.lib.x3d_apply_bfp(self._h, _op_id(name), self._bfp(out), self._bfp(inp)))
+
+    def step_bfp(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """The step on BFP32 fields."""
+        check("x3d_step_bfp", self.lib.x3d_step_bfp(
+            self._h, float(gdt), *(self._bfp(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
+
+    def step_timed_bfp(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
+        """step_timed for the BFP32 form; bytes per point are half the FP64 figures
+        (the Poisson figure leaves out its FP64 decode and encode)."""
+        from ._lib import X3D_NPASS
+        ms = (C.c_float * X3D_NPASS)()
+        check("x3d_step_timed_bfp", self.lib.x3d_step_timed_bfp(
+            self._h, float(gdt), *(self._bfp(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz)), ms))
+        return {self.lib.x3d_pass_name(i).decode(): (ms[i], self.lib.x3d_pass_bytes_per_point(i) / 2)
+                for i in range(X3D_NPASS)}
+
     def poisson_tables(self):
         hp = C.POINTER(C.c_double)
         a = [np.zeros(n) for n in (self.nx, self.nx, self.ny, self.ny, self.nz, self.nz)]

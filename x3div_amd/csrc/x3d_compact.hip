@@ -30,6 +30,14 @@ Circ<T> make_circ(double alfa, const LinePlan& P)
     c.pwM2[k] = (T)bM;
     bM = bM * bM;
   }
+  // beta^(M c) multiplied out of the binary powers in T, lowest bit first
+  // (the device's pow_Mc, so the table holds the same values)
+  for (int i = 0; i < 64; ++i) {
+    T r = T(1);
+    for (int k = 0; k < 7; ++k)
+      if ((i >> k) & 1) r = r * c.pwM2[k];
+    c.pMc[i] = r;
+  }
   c.pwMG = (T)std::pow(beta, (double)P.M * P.G);
   c.close = (T)(1.0 / (1.0 - std::pow(beta, (double)P.n)));
   return c;
@@ -50,35 +58,39 @@ Coef<double> stencil_coef(const Scheme& s, int kind)
 
 // The y/z circulant kernels of 16-point chunks are held to 80 VGPRs (they
 // need 78-80, no spills), so that three 512-thread work-groups share a
-// compute unit: 2-4% faster at 512^3 than two.
-template <typename T, int KIND, int M, bool CIRC, bool LF>
-__global__ __launch_bounds__(LF ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(LF && CIRC && M == 16 ? 6 : 1)))
-void k_line(const T* in, T* out, OpDev<T> op, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl)
+// compute unit: 2-4% faster at 512^3 than two.  BFP32 needs more (the
+// exponent of each loaded and stored row): 128.
+// S: the fields' storage (FP64, FP32 or BFP32, computed in FP64).
+template <typename S, int KIND, int M, bool CIRC, bool LF>
+__global__ __launch_bounds__(LF ? 1024 : 256) __attribute__((amdgpu_waves_per_eu(LF && CIRC && M == 16 ? (is_bfp<S> ? 4 : 6) : 1)))
+void k_line(const S* in, S* out, OpDev<typename Store<S>::T> op, Coef<typename Store<S>::T> cf,
+            Circ<typename Store<S>::T> cc, Geom geo, Tiling tl)
 {
+  using T = typename Store<S>::T;
   constexpr int HL = Halo<KIND>::L, HR = Halo<KIND>::R;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T>(geo, tl, M, tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x);
+  const LineThread t = line_thread<S>(geo, tl, M, tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x);
   T v[M];
   if constexpr (LF) {
     T win[M + HL + HR];
-    load_window<T, M, HL, HR, LF>(in, t, geo.n, win);
+    load_window<S, M, HL, HR, LF>(in, t, geo.n, win);
     rhs_chunk<KIND, T, M, HL>(win, cf, v);
     if constexpr (CIRC) solve_circ<T, M, LF>(v, t, cc, lds);
     else solve_table<T, M, LF>(v, t, op, lds, geo.n, tl.fix != 0);
-    store_chunk<T, M>(out, t, v);
+    store_chunk<S, M>(out, t, v);
   } else {
     // x-lines: the solvers use no LDS (a line is inside one wavefront), so
     // the LDS holds the wavefronts' staging tiles
-    const XBlock<T, M> xb = x_block<T, M>(geo, t, lds);
+    const XBlock<S, M> xb = x_block<S, M>(geo, t, lds);
     {
       T win[M + HL + HR];
-      x_load_window<T, M, HL, HR>(in, xb, t, win);
+      x_load_window<S, M, HL, HR>(in, xb, t, win);
       rhs_chunk<KIND, T, M, HL>(win, cf, v);
     }
     if constexpr (CIRC) solve_circ<T, M, LF>(v, t, cc, lds);
     else solve_table<T, M, LF>(v, t, op, lds, geo.n, tl.fix != 0);
-    x_store<T, M>(out, xb, v);
+    x_store<S, M>(out, xb, v);
   }
 }
 
@@ -103,6 +115,9 @@ int line_plan(int n, bool line_fast, LinePlan* P)
   pick(line_fast ? 32 : 64);
   if (!C && line_fast) pick(64);
   if (!C) return X3D_ERR_UNSUPPORTED;
+  // x-lines take their halo (at most 4 points) from the neighbouring chunks
+  // of the same wavefront, at most ceil(4 / M) of them, and wrap it once
+  if (!line_fast && (4 + M - 1) / M > C) return X3D_ERR_UNSUPPORTED;
   LinePlan p;
   p.n = n; p.M = M; p.C = C; p.line_fast = line_fast;
   if (line_fast) {
@@ -212,23 +227,23 @@ void launch_dims_plan(const x3d_ctx* c, int dir, const LinePlan& P, Geom* geo, T
   if (!P.line_fast) *smem = tbytes * (size_t)(P.Wc * P.Wl) * 64u * (size_t)((P.M & 1) ? P.M : P.M + 1);
 }
 
-template <typename T, int KIND, bool CIRC>
-static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStream_t st, const T* in, T* out,
+template <typename S, int KIND, bool CIRC, typename T = typename Store<S>::T>
+static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStream_t st, const S* in, S* out,
                               OpDev<T> op, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl)
 {
 #define X3D_CASE(MM)                                                                                   \
   case MM:                                                                                             \
     if (tl.line_fast)                                                                                  \
-      hipLaunchKernelGGL((k_line<T, KIND, MM, CIRC, true>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
+      hipLaunchKernelGGL((k_line<S, KIND, MM, CIRC, true>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
     else                                                                                               \
-      hipLaunchKernelGGL((k_line<T, KIND, MM, CIRC, false>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
+      hipLaunchKernelGGL((k_line<S, KIND, MM, CIRC, false>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
     break;
   // FP32 y/z lines also take the FP32 plans' 24- and 32-point chunks
 #define X3D_CASE_F32_YZ(MM)                                                                            \
   case MM:                                                                                             \
-    if constexpr (sizeof(T) == 4) {                                                                    \
+    if constexpr (std::is_same<S, float>::value) {                                                     \
       if (!tl.line_fast) return hipErrorInvalidValue;                                                  \
-      hipLaunchKernelGGL((k_line<T, KIND, MM, CIRC, true>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
+      hipLaunchKernelGGL((k_line<S, KIND, MM, CIRC, true>), grid, block, smem, st, in, out, op, cf, cc, geo, tl); \
       break;                                                                                           \
     } else {                                                                                           \
       return hipErrorInvalidValue;                                                                     \
@@ -244,17 +259,17 @@ static hipError_t launch_kind(int M, dim3 grid, dim3 block, size_t smem, hipStre
   return hipGetLastError();
 }
 
-template <typename T, bool CIRC>
-static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem, hipStream_t st, const T* in,
-                             T* out, OpDev<T> od, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl)
+template <typename S, bool CIRC, typename T = typename Store<S>::T>
+static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem, hipStream_t st, const S* in,
+                             S* out, OpDev<T> od, Coef<T> cf, Circ<T> cc, Geom geo, Tiling tl)
 {
   switch (kind) {
-    case K_D: return launch_kind<T, K_D, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_DVP: return launch_kind<T, K_DVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_DPV: return launch_kind<T, K_DPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_IVP: return launch_kind<T, K_IVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_IPV: return launch_kind<T, K_IPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
-    case K_DD: return launch_kind<T, K_DD, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_D: return launch_kind<S, K_D, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_DVP: return launch_kind<S, K_DVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_DPV: return launch_kind<S, K_DPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_IVP: return launch_kind<S, K_IVP, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_IPV: return launch_kind<S, K_IPV, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
+    case K_DD: return launch_kind<S, K_DD, CIRC>(M, grid, block, smem, st, in, out, od, cf, cc, geo, tl);
     default: return hipErrorInvalidValue;
   }
 }
@@ -339,6 +354,31 @@ int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, floa
   const Circ<float> cc = make_circ<float>(c->op[op].alfa, P);
   const OpDev<float> od{nullptr, nullptr, nullptr, nullptr, (float)c->op[op].alfa};
   const hipError_t e = launch_any<float, true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
+  return e == hipSuccess ? X3D_OK : hip_status(e);
+}
+
+// BFP32 form of one operator: BFP32 fields, the circulant solver in FP64 on
+// the FP64 plans (y/z-line lane groups of 16 lines: one segment per row).
+int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp32* out)
+{
+  if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS_ALL || !in || !out || kind == K_NONE) return X3D_ERR_ARG;
+  if (c->nx % 16) return X3D_ERR_UNSUPPORTED;
+  const LinePlan& P = c->plan[dir];
+  const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
+  if (c->op[op].n != n || P.n != n) return X3D_ERR_SIZE;
+  if (!c->op[op].circulant || P.M == 0) return X3D_ERR_UNSUPPORTED;
+  if (P.line_fast && P.LW % 16) return X3D_ERR_UNSUPPORTED;
+  Geom geo;
+  Tiling tl;
+  dim3 grid, block;
+  size_t smem;
+  launch_dims_plan(c, dir, P, &geo, &tl, &grid, &block, &smem, sizeof(double));
+  const Coef<double> k = stencil_coef(c->sc[dir], kind);
+  const double ig = 1.0 / circ_gamma(c->op[op].alfa);
+  const Coef<double> cf{k.a * ig, k.b * ig, k.c * ig, k.d * ig};
+  const Circ<double> cc = make_circ<double>(c->op[op].alfa, P);
+  const OpDev<double> od{nullptr, nullptr, nullptr, nullptr, c->op[op].alfa};
+  const hipError_t e = launch_any<bfp32, true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   return e == hipSuccess ? X3D_OK : hip_status(e);
 }
 

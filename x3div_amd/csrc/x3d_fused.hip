@@ -19,9 +19,11 @@ namespace x3d {
 // X: du_c = r_c; Y: tmp_c = du_c + r_c;
 // Z: du_c = tmp_c + r_c; if euler: u_c = gdt du_c + u_c.
 // (src/transeq.f90:100-115, 120-134, 139-153, 158-177 per component)
-template <typename T>
+// S: the fields' storage (FP64, FP32, or BFP32 computed in FP64).
+template <typename S>
 struct TranseqArgs {
-  T *ux, *uy, *uz, *dux, *duy, *duz, *tx, *ty, *tz;
+  using T = typename Store<S>::T;
+  S *ux, *uy, *uz, *dux, *duy, *duz, *tx, *ty, *tz;
   Coef<T> cf;
   Circ<T> cc;
   Geom geo;
@@ -30,12 +32,13 @@ struct TranseqArgs {
   int euler;
 };
 
-template <typename T, int M, int DIR>
-__device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ A, int blk)
+template <typename S, int M, int DIR>
+__device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ A, int blk)
 {
-  T* const ux = A.ux; T* const uy = A.uy; T* const uz = A.uz;
-  T* const dux = A.dux; T* const duy = A.duy; T* const duz = A.duz;
-  T* const tx = A.tx; T* const ty = A.ty; T* const tz = A.tz;
+  using T = typename Store<S>::T;
+  S* const ux = A.ux; S* const uy = A.uy; S* const uz = A.uz;
+  S* const dux = A.dux; S* const duy = A.duy; S* const duz = A.duz;
+  S* const tx = A.tx; S* const ty = A.ty; S* const tz = A.tz;
   const Coef<T> cf = A.cf;
   const Circ<T>& cc = A.cc;
   const Geom geo = A.geo;
@@ -44,16 +47,16 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
   constexpr int H = 2, NW = M + 2 * H;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T, DIR>(geo, A.tl, M, blk);
+  const LineThread t = line_thread<S, DIR>(geo, A.tl, M, blk);
   const int n = geo.n;
-  T* const Ud = DIR == 0 ? ux : (DIR == 1 ? uy : uz);
+  S* const Ud = DIR == 0 ? ux : (DIR == 1 ? uy : uz);
   // x: a second per-wavefront tile holds u_x (= u_d) for the whole kernel,
   // staged once; the first tile stages each u_c
-  XBlock<T, M> xb, xd;
+  XBlock<S, M> xb, xd;
   if constexpr (DIR == 0) {
-    xb = x_block<T, M>(geo, t, lds);
-    xd = x_block<T, M>(geo, t, lds + (blockDim.x >> 6) * x_tile_words<M>());
-    x_stage_in<T, M>(Ud, xd);
+    xb = x_block<S, M>(geo, t, lds);
+    xd = x_block<S, M>(geo, t, lds + (blockDim.x >> 6) * x_tile_words<M>());
+    x_stage_in<S, M>(Ud, xd);
   }
   // y/z: the u_d window (the same for the three components) and, with the
   // Euler update, each u_c chunk are kept in per-thread LDS slots from their
@@ -67,25 +70,25 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
     // components one after another: no loads hoisted across them
     __builtin_amdgcn_sched_barrier(0);
     // pointers straight from the kernel arguments (c is a constant here)
-    T* const Uc = c == 0 ? ux : (c == 1 ? uy : uz);
-    T* const DUc = c == 0 ? dux : (c == 1 ? duy : duz);
-    T* const TMc = c == 0 ? tx : (c == 1 ? ty : tz);
+    S* const Uc = c == 0 ? ux : (c == 1 ? uy : uz);
+    S* const DUc = c == 0 ? dux : (c == 1 ? duy : duz);
+    S* const TMc = c == 0 ? tx : (c == 1 ? ty : tz);
     // right-hand sides of D(u_c) and D(u_d u_c) first, so that only the chunk
     // values stay live through the two line solves
     T v1[M], v2[M];
     {
       T wc[NW], wd[NW];
       if constexpr (DIR == 0) {
-        x_window<T, M, H, H>(xd, t, wd);
+        x_window<S, M, H, H>(xd, t, wd);
         if (c == 0) {
 #pragma unroll
           for (int q = 0; q < NW; ++q) wc[q] = wd[q];
         } else {
-          x_load_window<T, M, H, H>(Uc, xb, t, wc);
+          x_load_window<S, M, H, H>(Uc, xb, t, wc);
         }
       } else {
         if (c == 0) {
-          load_window<T, M, H, H, true>(Ud, t, n, wd);
+          load_window<S, M, H, H, true>(Ud, t, n, wd);
 #pragma unroll
           for (int q = 0; q < NW; ++q) keep[q * nth + tid] = wd[q];
         } else {
@@ -96,7 +99,7 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
 #pragma unroll
           for (int q = 0; q < NW; ++q) wc[q] = wd[q];
         } else {
-          load_window<T, M, H, H, true>(Uc, t, n, wc);
+          load_window<S, M, H, H, true>(Uc, t, n, wc);
         }
         if constexpr (DIR == 2) {
 #pragma unroll
@@ -119,9 +122,9 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
     T a[M];
     if constexpr (PF) {
       if (t.active) {
-        const T* src = DIR == 1 ? DUc : TMc;
+        const S* src = DIR == 1 ? DUc : TMc;
 #pragma unroll
-        for (int k = 0; k < M; ++k) a[k] = ld(src, t.at(t.j0 + k));
+        for (int k = 0; k < M; ++k) a[k] = sld(src, t.at(t.j0 + k));
       }
     }
     // keep the two line solves apart: interleaving them doubles the live state
@@ -140,26 +143,26 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
       for (int k = 0; k < M; ++k) v1[k] = v1[k] + keep[(H + k) * nth + tid] * v2[k];
     }
     if constexpr (DIR == 0) {
-      x_store<T, M>(DUc, xb, v1);
+      x_store<S, M>(DUc, xb, v1);
     } else if constexpr (DIR == 1) {
       // tmp_c = du_c (x part) + r_c: the sum is formed here, where a strided
       // row stays inside one page, rather than in the z pass
       if (t.active) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) st(TMc, t.at(t.j0 + k), a[k] + v1[k]);
+        for (int k = 0; k < M; ++k) sst(TMc, t.at(t.j0 + k), a[k] + v1[k]);
       }
     } else {
       if (t.active) {
         if constexpr (!PF) {
 #pragma unroll
-          for (int k = 0; k < M; ++k) a[k] = ld(TMc, t.at(t.j0 + k));
+          for (int k = 0; k < M; ++k) a[k] = sld(TMc, t.at(t.j0 + k));
         }
 #pragma unroll
         for (int k = 0; k < M; ++k) {
           const uint32_t at = t.at(t.j0 + k);
           const T d = a[k] + v1[k];
-          st(DUc, at, d);
-          if (euler) st(Uc, at, gdt * d + keep[(NW + k) * nth + tid]);
+          sst(DUc, at, d);
+          if (euler) sst(Uc, at, gdt * d + keep[(NW + k) * nth + tid]);
         }
       }
     }
@@ -171,11 +174,12 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<T>& __restrict__ 
 // Y: o1 = Dvp(b) + Ivp(a), o2 = Ivp(c)             a, b, c = pp1, pgy1, pgz1
 // Z: o1 = Ivp(a) + Dvp(b)                          a, b = duydypi2, upi2
 // Arguments of the divergence and gradient passes.
-template <typename T>
+template <typename S>
 struct PassArgs {
-  const T *a, *b, *c;
-  T *o1, *o2, *o3;
-  T *ux, *uy, *uz;
+  using T = typename Store<S>::T;
+  const S *a, *b, *c;
+  S *o1, *o2, *o3;
+  S *ux, *uy, *uz;
   Coef<T> cfd;
   Circ<T> ccd;
   Coef<T> cfi;
@@ -185,76 +189,77 @@ struct PassArgs {
   int corr;
 };
 
-template <typename T, int M, int DIR>
-__device__ __forceinline__ void div_body(const PassArgs<T>& __restrict__ A, int blk)
+template <typename S, int M, int DIR>
+__device__ __forceinline__ void div_body(const PassArgs<S>& __restrict__ A, int blk)
 {
-  const T* const a = A.a; const T* const b = A.b; const T* const c = A.c;
-  T* const o1 = A.o1; T* const o2 = A.o2; T* const o3 = A.o3;
+  using T = typename Store<S>::T;
+  const S* const a = A.a; const S* const b = A.b; const S* const c = A.c;
+  S* const o1 = A.o1; S* const o2 = A.o2; S* const o3 = A.o3;
   const Coef<T> cfd = A.cfd, cfi = A.cfi;
   const Circ<T>& ccd = A.ccd;
   const Circ<T>& cci = A.cci;
   const Geom geo = A.geo;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T, DIR>(geo, A.tl, M, blk);
+  const LineThread t = line_thread<S, DIR>(geo, A.tl, M, blk);
   const int n = geo.n;
   if constexpr (DIR == 0) {
-    const XBlock<T, M> xb = x_block<T, M>(geo, t, lds);
+    const XBlock<S, M> xb = x_block<S, M>(geo, t, lds);
     T v[M];
     {
       T w[M + 3];
-      x_load_window<T, M, 1, 2>(a, xb, t, w);
+      x_load_window<S, M, 1, 2>(a, xb, t, w);
       rhs_chunk<K_DVP, T, M, 1>(w, cfd, v);
     }
     solve_circ<T, M, false>(v, t, ccd, lds);
-    x_store<T, M>(o1, xb, v);
+    x_store<S, M>(o1, xb, v);
     {
       T w[M + 7];
-      x_load_window<T, M, 3, 4>(b, xb, t, w);
+      x_load_window<S, M, 3, 4>(b, xb, t, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
     solve_circ<T, M, false>(v, t, cci, lds);
-    x_store<T, M>(o2, xb, v);
+    x_store<S, M>(o2, xb, v);
     {
       T w[M + 7];
-      x_load_window<T, M, 3, 4>(c, xb, t, w);
+      x_load_window<S, M, 3, 4>(c, xb, t, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
     solve_circ<T, M, false>(v, t, cci, lds);
-    x_store<T, M>(o3, xb, v);
+    x_store<S, M>(o3, xb, v);
   } else if constexpr (DIR == 1) {
     T v[M], s[M];
     {
       T w[M + 3];
-      load_window<T, M, 1, 2, DIR != 0>(b, t, n, w);
+      load_window<S, M, 1, 2, DIR != 0>(b, t, n, w);
       rhs_chunk<K_DVP, T, M, 1>(w, cfd, v);
     }
     solve_circ<T, M, DIR != 0>(v, t, ccd, lds);
     {
       T w[M + 7];
-      load_window<T, M, 3, 4, DIR != 0>(a, t, n, w);
+      load_window<S, M, 3, 4, DIR != 0>(a, t, n, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, s);
     }
     solve_circ<T, M, DIR != 0>(s, t, cci, lds);
 #pragma unroll
     for (int k = 0; k < M; ++k) v[k] = v[k] + s[k];
-    store_chunk<T, M>(o1, t, v);
+    store_chunk<S, M>(o1, t, v);
     {
       T w[M + 7];
-      load_window<T, M, 3, 4, DIR != 0>(c, t, n, w);
+      load_window<S, M, 3, 4, DIR != 0>(c, t, n, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
     solve_circ<T, M, DIR != 0>(v, t, cci, lds);
-    store_chunk<T, M>(o2, t, v);
+    store_chunk<S, M>(o2, t, v);
   } else {
     T v[M], s[M];
     {
       T w[M + 7];
-      load_window<T, M, 3, 4, DIR != 0>(a, t, n, w);
+      load_window<S, M, 3, 4, DIR != 0>(a, t, n, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
     T wb[M + 3];
-    load_window<T, M, 1, 2, DIR != 0>(b, t, n, wb);
+    load_window<S, M, 1, 2, DIR != 0>(b, t, n, wb);
     __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v, t, cci, lds);
     __builtin_amdgcn_sched_barrier(0);
@@ -262,7 +267,7 @@ __device__ __forceinline__ void div_body(const PassArgs<T>& __restrict__ A, int 
     solve_circ<T, M, DIR != 0>(s, t, ccd, lds);
 #pragma unroll
     for (int k = 0; k < M; ++k) v[k] = v[k] + s[k];
-    store_chunk<T, M>(o1, t, v);
+    store_chunk<S, M>(o1, t, v);
   }
 }
 
@@ -271,12 +276,13 @@ __device__ __forceinline__ void div_body(const PassArgs<T>& __restrict__ A, int 
 // Y: o1 = Ipv(a), o2 = Dpv(a), o3 = Ipv(b)          a, b = ppi3, pgz3
 // X: o1 = Dpv(a), o2 = Ipv(b), o3 = Ipv(c)          a, b, c = ppi2, pgy2, pgzi2
 //    if corr: u_k = u_k - o_k (src/navier.f90:84-86)
-template <typename T, int M, int DIR>
-__device__ __forceinline__ void grad_body(const PassArgs<T>& __restrict__ A, int blk)
+template <typename S, int M, int DIR>
+__device__ __forceinline__ void grad_body(const PassArgs<S>& __restrict__ A, int blk)
 {
-  const T* const a = A.a; const T* const b = A.b; const T* const c = A.c;
-  T* const o1 = A.o1; T* const o2 = A.o2; T* const o3 = A.o3;
-  T* const ux = A.ux; T* const uy = A.uy; T* const uz = A.uz;
+  using T = typename Store<S>::T;
+  const S* const a = A.a; const S* const b = A.b; const S* const c = A.c;
+  S* const o1 = A.o1; S* const o2 = A.o2; S* const o3 = A.o3;
+  S* const ux = A.ux; S* const uy = A.uy; S* const uz = A.uz;
   const Coef<T> cfd = A.cfd, cfi = A.cfi;
   const Circ<T>& ccd = A.ccd;
   const Circ<T>& cci = A.cci;
@@ -284,51 +290,51 @@ __device__ __forceinline__ void grad_body(const PassArgs<T>& __restrict__ A, int
   const int corr = A.corr;
   extern __shared__ unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const LineThread t = line_thread<T, DIR>(geo, A.tl, M, blk);
+  const LineThread t = line_thread<S, DIR>(geo, A.tl, M, blk);
   const int n = geo.n;
   if constexpr (DIR == 2 || DIR == 1) {
     T v[M];
     {
       T w[M + 7];
-      load_window<T, M, 4, 3, DIR != 0>(a, t, n, w);
+      load_window<S, M, 4, 3, DIR != 0>(a, t, n, w);
       rhs_chunk<K_IPV, T, M, 4>(w, cfi, v);
       T s[M];
       rhs_chunk<K_DPV, T, M, 4>(w, cfd, s);
       solve_circ<T, M, DIR != 0>(v, t, cci, lds);
-      store_chunk<T, M>(o1, t, v);
+      store_chunk<S, M>(o1, t, v);
       solve_circ<T, M, DIR != 0>(s, t, ccd, lds);
-      store_chunk<T, M>(o2, t, s);
+      store_chunk<S, M>(o2, t, s);
     }
     if constexpr (DIR == 1) {
       T w[M + 7];
-      load_window<T, M, 4, 3, DIR != 0>(b, t, n, w);
+      load_window<S, M, 4, 3, DIR != 0>(b, t, n, w);
       rhs_chunk<K_IPV, T, M, 4>(w, cfi, v);
       solve_circ<T, M, DIR != 0>(v, t, cci, lds);
-      store_chunk<T, M>(o3, t, v);
+      store_chunk<S, M>(o3, t, v);
     }
   } else {
-    const XBlock<T, M> xb = x_block<T, M>(geo, t, lds);
+    const XBlock<S, M> xb = x_block<S, M>(geo, t, lds);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const T* const inq = q == 0 ? a : (q == 1 ? b : c);
-      T* const outq = q == 0 ? o1 : (q == 1 ? o2 : o3);
-      T* const Uq = q == 0 ? ux : (q == 1 ? uy : uz);
+      const S* const inq = q == 0 ? a : (q == 1 ? b : c);
+      S* const outq = q == 0 ? o1 : (q == 1 ? o2 : o3);
+      S* const Uq = q == 0 ? ux : (q == 1 ? uy : uz);
       T v[M];
       {
         T w[M + 7];
-        x_load_window<T, M, 4, 3>(inq, xb, t, w);
+        x_load_window<S, M, 4, 3>(inq, xb, t, w);
         if (q == 0) rhs_chunk<K_DPV, T, M, 4>(w, cfd, v);
         else rhs_chunk<K_IPV, T, M, 4>(w, cfi, v);
       }
       if (q == 0) solve_circ<T, M, false>(v, t, ccd, lds);
       else solve_circ<T, M, false>(v, t, cci, lds);
-      x_store<T, M>(outq, xb, v);
+      x_store<S, M>(outq, xb, v);
       if (corr) {
         T u[M];
-        x_load_chunk<T, M>(Uq, xb, u);
+        x_load_chunk<S, M>(Uq, xb, u);
 #pragma unroll
         for (int k = 0; k < M; ++k) u[k] = u[k] - v[k];
-        x_store<T, M>(Uq, xb, u);
+        x_store<S, M>(Uq, xb, u);
       }
     }
   }
@@ -347,14 +353,14 @@ __device__ __forceinline__ int grid_of(const int* __restrict__ block0, int ngrid
   return lo;
 }
 
-template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) void k_transeq(TranseqArgs<T> A) { transeq_body<T, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
+template <typename S, int M, int DIR>
+__global__ __launch_bounds__(512) void k_transeq(TranseqArgs<S> A) { transeq_body<S, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
 // y passes: at most 128 VGPRs, so that two work-groups share a compute unit
 // (measured faster at 512^3 in both precisions despite a few spills)
-template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_div(PassArgs<T> A) { div_body<T, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
-template <typename T, int M, int DIR>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_grad(PassArgs<T> A) { grad_body<T, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
+template <typename S, int M, int DIR>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_div(PassArgs<S> A) { div_body<S, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
+template <typename S, int M, int DIR>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DIR == 1 ? 4 : 1))) void k_grad(PassArgs<S> A) { grad_body<S, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
 
 template <typename T, int M, int DIR>
 __global__ __launch_bounds__(512) void k_transeq_batch(const TranseqArgs<T>* __restrict__ list,
@@ -397,8 +403,9 @@ static void allow_lds(const void* fn, size_t bytes)
   if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename T>
+template <typename S>
 struct DirSet {
+  using T = typename Store<S>::T;
   Geom geo;
   Tiling tl;
   dim3 grid, block;
@@ -409,18 +416,20 @@ struct DirSet {
   Circ<T> cc_d, cc_s, cc_i;
 };
 
-// Plans: FP64 uses the fused plans, FP32 its own (longer chunks, 128-byte rows).
-template <typename T>
+// Plans: FP64 and BFP32 use the fused plans, FP32 its own (longer chunks,
+// 128-byte rows).
+template <typename S>
 static const LinePlan& fused_plan(const x3d_ctx* c, int dir)
 {
-  return sizeof(T) == 8 ? c->fplan[dir] : c->fplan32[dir];
+  return std::is_same<S, float>::value ? c->fplan32[dir] : c->fplan[dir];
 }
 
-template <typename T>
-static DirSet<T> dirset(const x3d_ctx* c, int dir)
+template <typename S>
+static DirSet<S> dirset(const x3d_ctx* c, int dir)
 {
-  DirSet<T> s;
-  const LinePlan& P = fused_plan<T>(c, dir);
+  using T = typename Store<S>::T;
+  DirSet<S> s;
+  const LinePlan& P = fused_plan<S>(c, dir);
   launch_dims_plan(c, dir, P, &s.geo, &s.tl, &s.grid, &s.block, &s.smem, sizeof(T));
   s.M = P.M;
   // keep slots of transeq: the u_d window (M + 4) and, along z, the u_c chunk
@@ -446,11 +455,11 @@ static DirSet<T> dirset(const x3d_ctx* c, int dir)
   return s;
 }
 
-// Chunk lengths instantiated per element type: FP64 up to 16 points, FP32 up
-// to 32 (the same registers per thread).
-#define X3D_MSWITCH_T(T, M, CALL_)                                                          \
+// Chunk lengths instantiated per storage: FP64 and BFP32 up to 16 points, FP32
+// up to 32 (the same registers per thread).
+#define X3D_MSWITCH_T(S, M, CALL_)                                                          \
   do {                                                                                      \
-    if constexpr (sizeof(T) == 8) {                                                         \
+    if constexpr (!std::is_same<S, float>::value) {                                         \
       switch (M) {                                                                          \
         case 1: { constexpr int MM = 1; CALL_; } break;                          \
         case 2: { constexpr int MM = 2; CALL_; } break;                          \
@@ -481,38 +490,52 @@ static DirSet<T> dirset(const x3d_ctx* c, int dir)
     }                                                                                       \
   } while (0)
 
-template <int DIR, typename T>
-static int transeq_pass(x3d_ctx* c, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz, T* tx, T* ty, T* tz, T gdt,
+// BFP32 fields: a y/z-line lane group must be 16 lines (one x-segment) or a
+// multiple, and x rows whole segments.
+template <typename S>
+static bool storage_ok(const x3d_ctx* c, int dir)
+{
+  if constexpr (!is_bfp<S>) return true;
+  const LinePlan& P = fused_plan<S>(c, dir);
+  return c->nx % 16 == 0 && (!P.line_fast || P.LW % 16 == 0);
+}
+
+template <int DIR, typename S>
+static int transeq_pass(x3d_ctx* c, S* ux, S* uy, S* uz, S* dux, S* duy, S* duz, S* tx, S* ty, S* tz, double gdt,
                         int euler)
 {
-  const DirSet<T> s = dirset<T>(c, DIR);
-  const TranseqArgs<T> A{ux, uy, uz, dux, duy, duz, tx, ty, tz, s.cf_d, s.cc_d, s.geo, s.tl, gdt, euler};
-  X3D_MSWITCH_T(T, s.M, {
-    allow_lds((const void*)k_transeq<T, MM, DIR>, s.smem_tq);
-    hipLaunchKernelGGL((k_transeq<T, MM, DIR>), s.grid, s.block, s.smem_tq, c->stream, A);
+  using T = typename Store<S>::T;
+  if (!storage_ok<S>(c, DIR)) return X3D_ERR_UNSUPPORTED;
+  const DirSet<S> s = dirset<S>(c, DIR);
+  const TranseqArgs<S> A{ux, uy, uz, dux, duy, duz, tx, ty, tz, s.cf_d, s.cc_d, s.geo, s.tl, (T)gdt, euler};
+  X3D_MSWITCH_T(S, s.M, {
+    allow_lds((const void*)k_transeq<S, MM, DIR>, s.smem_tq);
+    hipLaunchKernelGGL((k_transeq<S, MM, DIR>), s.grid, s.block, s.smem_tq, c->stream, A);
   });
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
 
-template <int DIR, typename T>
-static int div_pass(x3d_ctx* c, const T* a, const T* b, const T* cc, T* o1, T* o2, T* o3)
+template <int DIR, typename S>
+static int div_pass(x3d_ctx* c, const S* a, const S* b, const S* cc, S* o1, S* o2, S* o3)
 {
-  const DirSet<T> s = dirset<T>(c, DIR);
-  const PassArgs<T> A{a, b, cc, o1, o2, o3, nullptr, nullptr, nullptr, s.cf_dvp, s.cc_s, s.cf_ivp, s.cc_i,
+  if (!storage_ok<S>(c, DIR)) return X3D_ERR_UNSUPPORTED;
+  const DirSet<S> s = dirset<S>(c, DIR);
+  const PassArgs<S> A{a, b, cc, o1, o2, o3, nullptr, nullptr, nullptr, s.cf_dvp, s.cc_s, s.cf_ivp, s.cc_i,
                       s.geo, s.tl, 0};
-  X3D_MSWITCH_T(T, s.M, hipLaunchKernelGGL((k_div<T, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
+  X3D_MSWITCH_T(S, s.M, hipLaunchKernelGGL((k_div<S, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
 
-template <int DIR, typename T>
-static int grad_pass(x3d_ctx* c, const T* a, const T* b, const T* cc, T* o1, T* o2, T* o3, T* ux, T* uy, T* uz,
+template <int DIR, typename S>
+static int grad_pass(x3d_ctx* c, const S* a, const S* b, const S* cc, S* o1, S* o2, S* o3, S* ux, S* uy, S* uz,
                      int corr)
 {
-  const DirSet<T> s = dirset<T>(c, DIR);
-  const PassArgs<T> A{a, b, cc, o1, o2, o3, ux, uy, uz, s.cf_dpv, s.cc_s, s.cf_ipv, s.cc_i, s.geo, s.tl, corr};
-  X3D_MSWITCH_T(T, s.M, hipLaunchKernelGGL((k_grad<T, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
+  if (!storage_ok<S>(c, DIR)) return X3D_ERR_UNSUPPORTED;
+  const DirSet<S> s = dirset<S>(c, DIR);
+  const PassArgs<S> A{a, b, cc, o1, o2, o3, ux, uy, uz, s.cf_dpv, s.cc_s, s.cf_ipv, s.cc_i, s.geo, s.tl, corr};
+  X3D_MSWITCH_T(S, s.M, hipLaunchKernelGGL((k_grad<S, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
@@ -526,12 +549,13 @@ bool fused_ok(const x3d_ctx* c)
 
 #define X3D_TRY(expr) do { int _rc = (expr); if (_rc) return _rc; } while (0)
 
-// Scratch fields: the FP64 work arrays, viewed as T (FP32 uses half of each).
-template <typename T>
-static T* wk(x3d_ctx* c, int i) { return reinterpret_cast<T*>(c->work[i]); }
+// Scratch fields: the FP64 work arrays, viewed as S (FP32 and BFP32 use half
+// of each).
+template <typename S>
+static S* wk(x3d_ctx* c, int i) { return reinterpret_cast<S*>(c->work[i]); }
 
 template <typename T>
-int fused_momentum(x3d_ctx* c, T gdt, int euler, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz)
+int fused_momentum(x3d_ctx* c, double gdt, int euler, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz)
 {
   if (sizeof(T) == 8 && !c->f64_lines) return X3D_ERR_UNSUPPORTED;
   X3D_TRY(ensure_work(c));
@@ -711,12 +735,15 @@ template int batch_passes<double>(x3d_batch*, int);
 template int batch_passes<float>(x3d_batch*, int);
 
 template int fused_momentum<double>(x3d_ctx*, double, int, double*, double*, double*, double*, double*, double*);
-template int fused_momentum<float>(x3d_ctx*, float, int, float*, float*, float*, float*, float*, float*);
+template int fused_momentum<float>(x3d_ctx*, double, int, float*, float*, float*, float*, float*, float*);
+template int fused_momentum<bfp32>(x3d_ctx*, double, int, bfp32*, bfp32*, bfp32*, bfp32*, bfp32*, bfp32*);
 template int fused_divergence<double>(x3d_ctx*, double*, const double*, const double*, const double*);
 template int fused_divergence<float>(x3d_ctx*, float*, const float*, const float*, const float*);
+template int fused_divergence<bfp32>(x3d_ctx*, bfp32*, const bfp32*, const bfp32*, const bfp32*);
 template int fused_gradp<double>(x3d_ctx*, double*, double*, double*, const double*, double*, double*, double*,
                                  int);
 template int fused_gradp<float>(x3d_ctx*, float*, float*, float*, const float*, float*, float*, float*, int);
+template int fused_gradp<bfp32>(x3d_ctx*, bfp32*, bfp32*, bfp32*, const bfp32*, bfp32*, bfp32*, bfp32*, int);
 
 #undef X3D_MSWITCH_T
 

@@ -168,6 +168,12 @@ int line_plan_fused(int n, bool line_fast, LinePlan* out, int variant);
 
 // Launch one compact operator (or Thomas alone) along direction dir.
 int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out);
+// One word of a BFP32 field (block floating point: 31-bit mantissa, the
+// exponent of its 16-point x-segment spread over the segment's low bits).
+struct bfp32 {
+  int32_t w;
+};
+int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp32* out);
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
                 bool periodic_fix);
 // The Thomas sweeps alone (in place) with device tables f, s, w, p at
@@ -198,7 +204,7 @@ inline int mark_pass(x3d_ctx* c, int i)
 // Fused step passes (all operators must carry the library's periodic tables).
 bool fused_ok(const x3d_ctx* c);
 template <typename T>
-int fused_momentum(x3d_ctx* c, T gdt, int euler, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz);
+int fused_momentum(x3d_ctx* c, double gdt, int euler, T* ux, T* uy, T* uz, T* dux, T* duy, T* duz);
 template <typename T>
 int fused_divergence(x3d_ctx* c, T* pp3, const T* ux, const T* uy, const T* uz);
 template <typename T>

@@ -20,6 +20,8 @@
 //    rank-one correction.  It needs no per-point coefficient loads.
 #pragma once
 
+#include <type_traits>
+
 #include "x3d_internal.h"
 
 namespace x3d {
@@ -99,6 +101,7 @@ struct Circ {
   T pwMG;        // beta^(M G)
   T pwM2[7];     // beta^(M 2^k)
   T pw[33];      // beta^k, k = 0..M (M <= 32)
+  T pMc[64];     // beta^(M c), c = 0..63, formed as pow_Mc forms it
 };
 
 struct LineThread {
@@ -136,6 +139,92 @@ __device__ __forceinline__ void st(T* base, uint32_t boff, T v)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), field_rsrc(base), boff, 0, 0);
   else
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), field_rsrc(base), boff, 0, 0);
+}
+
+// ---- field storage ------------------------------------------------------------
+// A field is stored as FP64, FP32 or BFP32 elements (S); the line kernels
+// compute in Store<S>::T.
+//
+// BFP32 (block floating point, 4 bytes per point): x is cut into segments of
+// 16 points (nx a multiple of 16); the points of a segment share the exponent
+// e, the smallest with |v| < 2^e over the segment (clamped to [-126, 127]).
+// Each point is the 32-bit word (q << 1) | b with the 31-bit mantissa
+//   q = rint(v 2^(30-e)), clamped to [-2^30, 2^30 - 1],  v = q 2^(e-30),
+// so the error is at most 2^(e-31), and 2^(e-30) for values within half a unit
+// of 2^e: at most 2^-29 of the segment's largest value (FP32 rounds each value
+// to 2^-24 of itself); b, the low bit, is bit k of
+// e + 128 in the segment's point k for k < 8 (0 in points 8-15).  A field is
+// thus one int32 array of the grid's shape and the exponent travels with the
+// mantissas.  The kernels hold a segment in one aligned 16-lane row of a
+// wavefront (the 16 consecutive y/z-lines of a lane group, or 16 consecutive
+// staged x points), read the exponent with one ballot and form it on store
+// with a row maximum; every lane of a row must take part in both.
+template <typename S>
+struct Store { using T = S; };
+template <>
+struct Store<bfp32> { using T = double; };
+template <typename S>
+inline constexpr bool is_bfp = std::is_same<S, bfp32>::value;
+
+// Value of this lane's word; the exponent comes from the words of the lane's
+// 16-lane row.
+__device__ __forceinline__ double bfp_value(int w)
+{
+  const unsigned long long m = __ballot(w & 1);
+  const int eb = (int)((m >> (threadIdx.x & 48)) & 0xFFull);
+  return ldexp((double)(w >> 1), eb - 158);  // (w >> 1) 2^((eb - 128) - 30)
+}
+
+// Largest value over this lane's 16-lane row (DPP row rotations; every lane
+// of the row is active, so no lane reads a disabled one).
+__device__ __forceinline__ unsigned row16_max(unsigned x)
+{
+  unsigned y;
+  y = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, true);  // row_ror:8
+  x = x > y ? x : y;
+  y = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x124, 0xF, 0xF, true);  // row_ror:4
+  x = x > y ? x : y;
+  y = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x122, 0xF, 0xF, true);  // row_ror:2
+  x = x > y ? x : y;
+  y = (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x121, 0xF, 0xF, true);  // row_ror:1
+  return x > y ? x : y;
+}
+
+// Word of v in this lane's row: the row's common exponent, then v's mantissa
+// and this lane's exponent bit.  The exponent comes from the largest FP64
+// exponent field in the row (high word without the sign): e = biased - 1022
+// is the smallest with |v| < 2^e for normal values; zero and FP64 subnormals
+// give -1022 and clamp to -126.  |v| < 2^e bounds the rounded mantissa by
+// 2^30, the one value the clamp changes.
+__device__ __forceinline__ int bfp_word(double v)
+{
+  const unsigned hi = (unsigned)__double2hiint(v) & 0x7FFFFFFFu;
+  int e = (int)(row16_max(hi) >> 20) - 1022;
+  e = min(max(e, -126), 127);
+  int q = (int)rint(ldexp(v, 30 - e));
+  q = min(max(q, -1073741824), 1073741823);
+  const unsigned bit = ((unsigned)(e + 128) >> (threadIdx.x & 15)) & 1u;  // 0 in points 8-15: e + 128 < 256
+  return (int)(((unsigned)q << 1) | bit);
+}
+
+template <typename S>
+__device__ __forceinline__ typename Store<S>::T sld(const S* base, uint32_t boff)
+{
+  if constexpr (is_bfp<S>)
+    return bfp_value(ld(reinterpret_cast<const int*>(base), boff));
+  else
+    return ld(base, boff);
+}
+// With BFP32 every lane of the row calls this (on or not).
+template <typename S>
+__device__ __forceinline__ void sst(S* base, uint32_t boff, typename Store<S>::T v, bool on = true)
+{
+  if constexpr (is_bfp<S>) {
+    const int w = bfp_word(v);
+    if (on) st(reinterpret_cast<int*>(base), boff, w);
+  } else {
+    if (on) st(base, boff, v);
+  }
 }
 
 // DIRC >= 0 fixes the direction at compile time; otherwise geo.dir selects it.
@@ -179,10 +268,12 @@ __device__ __forceinline__ LineThread line_thread(const Geom& geo, const Tiling&
 // x-lines (consecutive lanes hold consecutive chunks): the M chunk points are
 // loaded from one address with immediate offsets and the halo comes from the
 // neighbouring lanes; strided y/z-lines load every point.
-template <typename T, int M, int HL, int HR, bool LF>
-__device__ __forceinline__ void load_window(const T* __restrict__ in, const LineThread& t, int n,
-                                            T (&win)[M + HL + HR])
+template <typename S, int M, int HL, int HR, bool LF>
+__device__ __forceinline__ void load_window(const S* __restrict__ in, const LineThread& t, int n,
+                                            typename Store<S>::T (&win)[M + HL + HR])
 {
+  using T = typename Store<S>::T;
+  static_assert(LF || !is_bfp<S>, "BFP32 x-lines go through the staged tile");
   if constexpr (!LF) {
     const uint32_t o = t.at(t.j0);
 #pragma unroll
@@ -207,7 +298,7 @@ __device__ __forceinline__ void load_window(const T* __restrict__ in, const Line
     // in the groups at the wavefront's ends (their neighbours live in other
     // wavefronts), so the masked loads touch a quarter of the rows
 #pragma unroll
-    for (int k = 0; k < M; ++k) win[HL + k] = ld(in, t.at(t.j0 + k));
+    for (int k = 0; k < M; ++k) win[HL + k] = sld(in, t.at(t.j0 + k));
 #pragma unroll
     for (int h = 1; h <= HL; ++h) {
       const int d = (h + M - 1) / M;  // chunks to the left
@@ -216,7 +307,7 @@ __device__ __forceinline__ void load_window(const T* __restrict__ in, const Line
       if (t.g < d) {
         int j = t.j0 - h;
         j = j < 0 ? j + n : j;
-        v = ld(in, t.at(j));
+        v = sld(in, t.at(j));
       }
       win[HL - h] = v;
     }
@@ -228,26 +319,28 @@ __device__ __forceinline__ void load_window(const T* __restrict__ in, const Line
       if (t.g + d >= t.G) {
         int j = t.j0 + M - 1 + h;
         j = j >= n ? j - n : j;
-        v = ld(in, t.at(j));
+        v = sld(in, t.at(j));
       }
       win[HL + M - 1 + h] = v;
     }
   }
 }
 
-template <typename T, int M>
-__device__ __forceinline__ void load_chunk(const T* __restrict__ in, const LineThread& t, T (&v)[M])
+template <typename S, int M>
+__device__ __forceinline__ void load_chunk(const S* __restrict__ in, const LineThread& t,
+                                           typename Store<S>::T (&v)[M])
 {
 #pragma unroll
-  for (int k = 0; k < M; ++k) v[k] = ld(in, t.at(t.j0 + k));
+  for (int k = 0; k < M; ++k) v[k] = sld(in, t.at(t.j0 + k));
 }
 
-template <typename T, int M>
-__device__ __forceinline__ void store_chunk(T* out, const LineThread& t, const T (&v)[M])
+// (y/z-lines: a 16-lane row holds 16 consecutive lines, all active or all not)
+template <typename S, int M>
+__device__ __forceinline__ void store_chunk(S* out, const LineThread& t, const typename Store<S>::T (&v)[M])
 {
   if (t.active) {
 #pragma unroll
-    for (int k = 0; k < M; ++k) st(out, t.at(t.j0 + k), v[k]);
+    for (int k = 0; k < M; ++k) sst(out, t.at(t.j0 + k), v[k]);
   }
 }
 
@@ -274,9 +367,9 @@ struct XPitch { static constexpr int P = (M & 1) ? M : M + 1; };
 template <int M>
 __host__ __device__ constexpr int x_tile_words() { return 64 * XPitch<M>::P; }
 
-template <typename T, int M>
+template <typename S, int M>
 struct XBlock {
-  T* tile;        // this wavefront's LDS tile [64][P]
+  typename Store<S>::T* tile;  // this wavefront's LDS tile [64][P]
   uint32_t base;  // byte offset of the block's first point
   uint32_t lim;   // byte offset of the field's last point (clamp for partial blocks)
   int valid;      // points of the block that belong to existing lines
@@ -284,18 +377,18 @@ struct XBlock {
 };
 
 // tiles: the work-group's x tiles, one per wavefront, in wavefront order.
-template <typename T, int M>
-__device__ __forceinline__ XBlock<T, M> x_block(const Geom& geo, const LineThread& t, T* tiles)
+template <typename S, int M>
+__device__ __forceinline__ XBlock<S, M> x_block(const Geom& geo, const LineThread& t, typename Store<S>::T* tiles)
 {
-  XBlock<T, M> b;
+  XBlock<S, M> b;
   const int wave = threadIdx.x >> 6;
   b.lane = threadIdx.x & 63;
   b.tile = tiles + wave * x_tile_words<M>();
   const int first = t.line - t.lineLocal;  // first line of this wavefront
   const long long left = (long long)(geo.nlines - first) * geo.n;
   b.valid = left <= 0 ? 0 : (left >= 64 * M ? 64 * M : (int)left);
-  b.base = (uint32_t)first * (uint32_t)geo.n * (uint32_t)sizeof(T);
-  b.lim = ((uint32_t)geo.nlines * (uint32_t)geo.n - 1u) * (uint32_t)sizeof(T);
+  b.base = (uint32_t)first * (uint32_t)geo.n * (uint32_t)sizeof(S);
+  b.lim = ((uint32_t)geo.nlines * (uint32_t)geo.n - 1u) * (uint32_t)sizeof(S);
   return b;
 }
 
@@ -334,12 +427,15 @@ __device__ __forceinline__ void st2(T* base, uint32_t boff, T a, T b)
 // row e / M, column e % M).  With M even each lane moves two neighbouring
 // points per access (both in one chunk); lines have even length then, so
 // pairs never straddle the end of the field.
-template <typename T, int M>
-__device__ __forceinline__ void x_stage_in(const T* __restrict__ in, const XBlock<T, M>& b)
+// BFP32 moves one point per lane and access, so that a 16-lane row holds one
+// segment.
+template <typename S, int M>
+__device__ __forceinline__ void x_stage_in(const S* __restrict__ in, const XBlock<S, M>& b)
 {
+  using T = typename Store<S>::T;
   constexpr int P = XPitch<M>::P;
   wave_lds_sync();
-  if constexpr (M % 2 == 0) {
+  if constexpr (M % 2 == 0 && !is_bfp<S>) {
     const uint32_t last = b.lim - (uint32_t)sizeof(T);  // first point of the field's last pair
 #pragma unroll
     for (int k = 0; k < M / 2; ++k) {
@@ -356,9 +452,9 @@ __device__ __forceinline__ void x_stage_in(const T* __restrict__ in, const XBloc
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const int e = b.lane + 64 * k;
-      uint32_t off = b.base + (uint32_t)e * (uint32_t)sizeof(T);
+      uint32_t off = b.base + (uint32_t)e * (uint32_t)sizeof(S);
       off = off < b.lim ? off : b.lim;
-      b.tile[(e / M) * P + e % M] = ld(in, off);
+      b.tile[(e / M) * P + e % M] = sld(in, off);
     }
   }
   wave_lds_sync();
@@ -366,56 +462,71 @@ __device__ __forceinline__ void x_stage_in(const T* __restrict__ in, const XBloc
 
 // Window j0-HL .. j0+M+HR-1 of this lane's chunk from the staged tile; the
 // halo is in the neighbouring chunks of the same line (wrapped mod n).
-template <typename T, int M, int HL, int HR>
-__device__ __forceinline__ void x_window(const XBlock<T, M>& b, const LineThread& t, T (&win)[M + HL + HR])
+template <typename S, int M, int HL, int HR>
+__device__ __forceinline__ void x_window(const XBlock<S, M>& b, const LineThread& t,
+                                         typename Store<S>::T (&win)[M + HL + HR])
 {
+  using T = typename Store<S>::T;
   constexpr int P = XPitch<M>::P;
   const T* own = b.tile + b.lane * P;
 #pragma unroll
   for (int k = 0; k < M; ++k) win[HL + k] = own[k];
+  // the chunk d to the left / right, wrapped: the x plans keep the halo
+  // inside the neighbouring ceil(4 / M) <= G chunks, so one correction does
 #pragma unroll
   for (int h = 1; h <= HL; ++h) {
     const int d = (h + M - 1) / M;
-    const int src = t.lane_of((t.g - d % t.G + t.G) % t.G);
+    const int gl = t.g - d;
+    const int src = t.lane_of(gl < 0 ? gl + t.G : gl);
     win[HL - h] = b.tile[src * P + (M * d - h)];
   }
 #pragma unroll
   for (int h = 1; h <= HR; ++h) {
     const int d = (h + M - 1) / M;
-    const int src = t.lane_of((t.g + d) % t.G);
+    const int gr = t.g + d;
+    const int src = t.lane_of(gr >= t.G ? gr - t.G : gr);
     win[HL + M - 1 + h] = b.tile[src * P + (h - 1 - M * (d - 1))];
   }
 }
 
-template <typename T, int M, int HL, int HR>
-__device__ __forceinline__ void x_load_window(const T* __restrict__ in, const XBlock<T, M>& b,
-                                              const LineThread& t, T (&win)[M + HL + HR])
+template <typename S, int M, int HL, int HR>
+__device__ __forceinline__ void x_load_window(const S* __restrict__ in, const XBlock<S, M>& b,
+                                              const LineThread& t, typename Store<S>::T (&win)[M + HL + HR])
 {
-  x_stage_in<T, M>(in, b);
-  x_window<T, M, HL, HR>(b, t, win);
+  x_stage_in<S, M>(in, b);
+  x_window<S, M, HL, HR>(b, t, win);
 }
 
 // This lane's chunk of a field, through the tile.
-template <typename T, int M>
-__device__ __forceinline__ void x_load_chunk(const T* __restrict__ in, const XBlock<T, M>& b, T (&v)[M])
+template <typename S, int M>
+__device__ __forceinline__ void x_load_chunk(const S* __restrict__ in, const XBlock<S, M>& b,
+                                             typename Store<S>::T (&v)[M])
 {
-  x_stage_in<T, M>(in, b);
+  using T = typename Store<S>::T;
+  x_stage_in<S, M>(in, b);
   const T* own = b.tile + b.lane * XPitch<M>::P;
 #pragma unroll
   for (int k = 0; k < M; ++k) v[k] = own[k];
 }
 
 // Chunk values to the tile, then coalesced stores of the block's valid points.
-template <typename T, int M>
-__device__ __forceinline__ void x_store(T* out, const XBlock<T, M>& b, const T (&v)[M])
+template <typename S, int M>
+__device__ __forceinline__ void x_store(S* out, const XBlock<S, M>& b, const typename Store<S>::T (&v)[M])
 {
+  using T = typename Store<S>::T;
   constexpr int P = XPitch<M>::P;
   wave_lds_sync();
   T* own = b.tile + b.lane * P;
 #pragma unroll
   for (int k = 0; k < M; ++k) own[k] = v[k];
   wave_lds_sync();
-  if constexpr (M % 2 == 0) {
+  if constexpr (is_bfp<S>) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = b.lane + 64 * k;
+      sst(out, b.base + (uint32_t)e * (uint32_t)sizeof(S), b.tile[(e / M) * P + e % M], e < b.valid);
+    }
+  } else if constexpr (M % 2 == 0) {
 #pragma unroll
     for (int k = 0; k < M / 2; ++k) {
       const int e = 2 * b.lane + 128 * k;
@@ -488,7 +599,10 @@ __device__ __forceinline__ void solve_circ(T (&v)[M], const LineThread& t, const
     total = __shfl(E, t.lane_of(G - 1));
   }
   const T ylast = total * cc.close;
-  const T carry = pow_Mc(cc, t.c) * ylast + (Eex + pow_Mc(cc, g) * cw);
+  // beta^(M c): x-lines take it from the table in the kernel arguments (one
+  // load rather than seven selected multiplies); y/z-lines form it, as the
+  // loads would hold registers their work-groups of up to 16 wavefronts lack
+  const T carry = (LF ? pow_Mc(cc, t.c) : cc.pMc[t.c]) * ylast + (Eex + (LF ? pow_Mc(cc, g) : cc.pMc[g]) * cw);
 #pragma unroll
   for (int k = 0; k < M; ++k) v[k] = v[k] + cc.pw[k + 1] * carry;
 
@@ -520,7 +634,8 @@ __device__ __forceinline__ void solve_circ(T (&v)[M], const LineThread& t, const
     total2 = __shfl(F, t.lane_of(0));
   }
   const T x0 = total2 * cc.close;
-  const T carry2 = (Fex + pow_Mc(cc, G - 1 - g) * cw2) + pow_Mc(cc, t.C - 1 - t.c) * x0;
+  const T carry2 = (Fex + (LF ? pow_Mc(cc, G - 1 - g) : cc.pMc[G - 1 - g]) * cw2) +
+                   (LF ? pow_Mc(cc, t.C - 1 - t.c) : cc.pMc[t.C - 1 - t.c]) * x0;
 #pragma unroll
   for (int k = 0; k < M; ++k) v[k] = v[k] + cc.pw[M - k] * carry2;
 }
