@@ -14,7 +14,9 @@
  *  - Outputs are fully overwritten and inputs are not modified (except the
  *    in-place Thomas and Poisson calls), as in the reference.
  *  - Boundary conditions are triply periodic (ncl?1 = ncl?n = 0), the only
- *    case the reference's Poisson solver accepts (src/poisson.f90:104-108).
+ *    case the reference's Poisson solver accepts (src/poisson.f90:104-108),
+ *    except on a context made by x3d_create_bc, whose non-periodic
+ *    directions take the first derivatives der?_11/12/21/22 only.
  *  - Suffix _f32 marks the precision form: FP32 fields, FP32 line solves.
  *
  * Reference citations are relative to the x3div source tree (src/...).
@@ -72,6 +74,21 @@ const char *x3d_strerror(int code);
  * x3d_derive_init (src/x3d_derive.f90:66-93) and decomp_2d_poisson_init
  * (src/poisson.f90:73-231), on the current HIP device. */
 int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_ctx **ctx);
+/* The same with the boundary codes of parameter() (nclx1/nclxn, ncly1/nclyn,
+ * nclz1/nclzn, src/module_param.f90:88; src/parameters.f90): per direction
+ * 0 periodic at both ends, or 1 (free-slip) / 2 (Dirichlet) at each end, and
+ * dX = L / (nX - 1) in a non-periodic direction.  NULL arrays mean periodic.
+ * In a non-periodic direction the first-derivative instances (derX: npaire
+ * 0, derXp: npaire 1) get first_deriv_imp_'s non-periodic tables
+ * (src/x3d_operator_1d.f90:395-491) and x3d_der{x,y,z} runs der?_11/12/21/22
+ * (src/x3d_derive.f90:156-275, 343-499, 562-725); the Thomas entry points run
+ * the _12 sweeps for them.  That direction's staggered, interpolation and
+ * second-derivative instances, and on such a grid the Poisson solver, the
+ * divergence, gradp, the momentum right-hand side and the step, return
+ * X3D_ERR_UNSUPPORTED (the reference's Poisson stops there,
+ * src/poisson.f90:104-108). */
+int x3d_create_bc(int nx, int ny, int nz, double xlx, double yly, double zlz,
+                  const int *ncl1, const int *ncln, x3d_ctx **ctx);
 /* x3d_operator_1d_finalize + decomp_2d_poisson_finalize (src/poisson.f90:237-271) */
 int x3d_destroy(x3d_ctx *ctx);
 int x3d_set_stream(x3d_ctx *ctx, void *hip_stream);

@@ -83,6 +83,11 @@ def lib(fast: bool = False):
         L.orc_test_err.restype = C.c_double
         L.orc_test_err.argtypes = [vp, dp, C.c_int]
         L.orc_div_stats.argtypes = [vp, dp, dp, dp]
+        L.orc_np_consts.argtypes = [C.c_double, dp]
+        L.orc_np_tables.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int] + [dp] * 8
+        L.orc_np_tables.restype = C.c_int
+        L.orc_der_np.argtypes = [dp, dp] + [C.c_int] * 4 + [C.c_double] + [C.c_int] * 3 + [dp] * 3
+        L.orc_der_np.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
         _libs[fast] = L
@@ -92,6 +97,46 @@ def lib(fast: bool = False):
 def _p(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def np_tables(n: int, h: float, ncl1: int, ncln: int):
+    """Non-periodic first-derivative tables of first_deriv_imp_
+    (src/x3d_operator_1d.f90:395-491): {"f", "s", "w"} of the npaire = 0
+    instance (derX), {"fp", "sp", "wp"} of the npaire = 1 one (derXp), and
+    the rows (lower, diagonal, upper) each was prepared from."""
+    L = lib(False)
+    a = {k: np.zeros(n) for k in ("f", "s", "w", "fp", "sp", "wp")}
+    rows, rowsp = np.zeros(3 * n), np.zeros(3 * n)
+    if L.orc_np_tables(n, h, ncl1, ncln, *(_p(a[k]) for k in ("f", "s", "w", "fp", "sp", "wp")),
+                       _p(rows), _p(rowsp)):
+        raise ValueError("oracle: bad non-periodic table request")
+    a["rows"] = rows.reshape(3, n)
+    a["rowsp"] = rowsp.reshape(3, n)
+    return a
+
+
+def np_consts(h: float):
+    """alfai afi bfi alfa1 af1 bf1 cf1 alfa2 af2 alfan afn bfn cfn alfam afm."""
+    k = np.zeros(15)
+    lib(False).orc_np_consts(h, _p(k))
+    return k
+
+
+def der_np(u: np.ndarray, axis: str, L: float, ncl1: int, ncln: int, npaire: int) -> np.ndarray:
+    """der{x,y,z}_{ncl1}{ncln} of src/x3d_derive.f90:156-275 (and its y, z
+    copies) on a (nz, ny, nx) field, grid spacing L / (n - 1), with the
+    npaire instance's tables."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    nz, ny, nx = u.shape
+    d = "xyz".index(axis)
+    n = (nx, ny, nz)[d]
+    h = L / (n - 1)
+    tb = np_tables(n, h, ncl1, ncln)
+    f, s, w = (tb["fp"], tb["sp"], tb["wp"]) if npaire == 1 else (tb["f"], tb["s"], tb["w"])
+    t = np.zeros_like(u)
+    if lib(False).orc_der_np(_p(t), _p(u), d, nx, ny, nz, h, ncl1, ncln, npaire, _p(f), _p(s), _p(w)):
+        raise ValueError("oracle: bad non-periodic derivative request")
+    return t
 
 
 class Oracle:

@@ -527,6 +527,125 @@ void orc_derz(const orc_ctx *c, double *t, const double *u, int op, int nx, int 
 { orc_apply(c, t, u, op, 2, 0, nx, ny, nz); }
 
 /* ------------------------------------------------------------------------ */
+/* Non-periodic first derivatives: der?_11/12/21/22                          */
+/* (src/x3d_derive.f90:156-275, and the y/z copies :343-499, :562-725).      */
+/* Boundary codes: 1 free-slip (npaire chooses the even or odd extension),   */
+/* 2 Dirichlet (one-sided compact closure).  The grid spacing is L/(n-1).    */
+/* ------------------------------------------------------------------------ */
+/* first_deriv_exp_, ifirstder = 4 (src/x3d_operator_1d.f90:325-393) */
+void orc_np_consts(double h, double *k /* 15 */)
+{
+  k[0] = 1.0 / 3.0;            /* alfai */
+  k[1] = (7.0 / 9.0) / h;      /* afi   */
+  k[2] = (1.0 / 36.0) / h;     /* bfi   */
+  k[3] = 2.0;                  /* alfa1 */
+  k[4] = -(5.0 / 2.0) / h;     /* af1   */
+  k[5] = 2.0 / h;              /* bf1   */
+  k[6] = 0.5 / h;              /* cf1   */
+  k[7] = 1.0 / 4.0;            /* alfa2 */
+  k[8] = (3.0 / 4.0) / h;      /* af2   */
+  k[9] = 2.0;                  /* alfan */
+  k[10] = -(5.0 / 2.0) / h;    /* afn   */
+  k[11] = 2.0 / h;             /* bfn   */
+  k[12] = 0.5 / h;             /* cfn   */
+  k[13] = 1.0 / 4.0;           /* alfam */
+  k[14] = (3.0 / 4.0) / h;     /* afm   */
+}
+
+/* first_deriv_imp_ for ncl1, ncln in {1, 2} (src/x3d_operator_1d.f90:395-491):
+ * the rows (lower fb, diagonal fc, upper ff) and the Thomas vectors of the
+ * npaire = 0 instance (ff, fs, fw) and of the npaire = 1 instance (ffp, fsp,
+ * fwp).  rows (3n, optional) receives fb, fc, ff of the first, rowsp those of
+ * the second. */
+int orc_np_tables(int n, double h, int ncl1, int ncln, double *ff, double *fs, double *fw,
+                  double *ffp, double *fsp, double *fwp, double *rows, double *rowsp)
+{
+  if (n < 8 || ncl1 < 1 || ncl1 > 2 || ncln < 1 || ncln > 2) return -1;
+  double k[15];
+  orc_np_consts(h, k);
+  const double alfai = k[0], alfa1 = k[3], alfa2 = k[7], alfan = k[9], alfam = k[13];
+  double *fb = calloc(n, sizeof(double)), *fc = calloc(n, sizeof(double));
+  if (!fb || !fc) { free(fb); free(fc); return -2; }
+  for (int i = 0; i < n; ++i) { ff[i] = fs[i] = fw[i] = ffp[i] = fsp[i] = fwp[i] = 0.0; }
+  if (ncl1 == 1) {
+    ff[0] = alfai + alfai; ff[1] = alfai; fc[0] = 1.0; fc[1] = 1.0; fb[0] = alfai; fb[1] = alfai;
+  } else {
+    ff[0] = alfa1; ff[1] = alfa2; fc[0] = 1.0; fc[1] = 1.0; fb[0] = alfa2; fb[1] = alfai;
+  }
+  if (ncln == 1) {
+    ff[n - 3] = alfai; ff[n - 2] = alfai; ff[n - 1] = 0.0;
+    fc[n - 3] = 1.0; fc[n - 2] = 1.0; fc[n - 1] = 1.0;
+    fb[n - 3] = alfai; fb[n - 2] = alfai + alfai; fb[n - 1] = 0.0;
+  } else {
+    ff[n - 3] = alfai; ff[n - 2] = alfam; ff[n - 1] = 0.0;
+    fc[n - 3] = 1.0; fc[n - 2] = 1.0; fc[n - 1] = 1.0;
+    fb[n - 3] = alfam; fb[n - 2] = alfan; fb[n - 1] = 0.0;
+  }
+  for (int i = 2; i <= n - 4; ++i) { ff[i] = alfai; fc[i] = 1.0; fb[i] = alfai; }
+  for (int i = 0; i < n; ++i) ffp[i] = ff[i];
+  if (rows) { memcpy(rows, fb, sizeof(double) * n); memcpy(rows + n, fc, sizeof(double) * n); memcpy(rows + 2 * n, ff, sizeof(double) * n); }
+  orc_prepare(fb, fc, ff, fs, fw, n);
+  if (ncl1 == 1) ffp[0] = 0.0;
+  if (ncln == 1) fb[n - 2] = 0.0;
+  if (rowsp) { memcpy(rowsp, fb, sizeof(double) * n); memcpy(rowsp + n, fc, sizeof(double) * n); memcpy(rowsp + 2 * n, ffp, sizeof(double) * n); }
+  orc_prepare(fb, fc, ffp, fsp, fwp, n);
+  free(fb); free(fc);
+  return 0;
+}
+
+/* The right-hand side of derx_ij at point i of one line u[i * stride]. */
+static double np_rhs(const double *u, long stride, int i, int n, int ncl1, int ncln, int npaire,
+                     const double *k)
+{
+#define V(o) u[(long)(i + (o)) * stride]
+  const double afi = k[1], bfi = k[2];
+  if (i >= 2 && i <= n - 3) return afi * (V(1) - V(-1)) + bfi * (V(2) - V(-2));
+  if (i == 0) {
+    if (ncl1 == 1) return npaire == 1 ? 0.0 : afi * (V(1) + V(1)) + bfi * (V(2) + V(2));
+    return k[4] * V(0) + k[5] * V(1) + k[6] * V(2);
+  }
+  if (i == 1) {
+    if (ncl1 == 1)
+      return npaire == 1 ? afi * (V(1) - V(-1)) + bfi * (V(2) - V(0)) : afi * (V(1) - V(-1)) + bfi * (V(2) + V(0));
+    return k[8] * (V(1) - V(-1));
+  }
+  if (i == n - 2) {
+    if (ncln == 1)
+      return npaire == 1 ? afi * (V(1) - V(-1)) + bfi * (V(0) - V(-2)) : afi * (V(1) - V(-1)) + bfi * ((-V(0)) - V(-2));
+    return k[14] * (V(1) - V(-1));
+  }
+  /* i == n - 1 */
+  if (ncln == 1) return npaire == 1 ? 0.0 : afi * ((-V(-1)) - V(-1)) + bfi * ((-V(-2)) - V(-2));
+  return -k[10] * V(0) - k[11] * V(-1) - k[12] * V(-2);
+#undef V
+}
+
+/* derx_ij / dery_ij / derz_ij: the right-hand side above on every line along
+ * dir, then the non-periodic Thomas sweeps (xthomas_12, src/thomas.f90:61-84)
+ * with the instance's tables (ff, fs, fw). */
+int orc_der_np(double *t, const double *u, int dir, int nx, int ny, int nz, double h, int ncl1,
+               int ncln, int npaire, const double *ff, const double *fs, const double *fw)
+{
+  if (dir < 0 || dir > 2) return -1;
+  const int n = dir == 0 ? nx : dir == 1 ? ny : nz;
+  if (n < 8 || ncl1 < 1 || ncl1 > 2 || ncln < 1 || ncln > 2) return -1;
+  double k[15];
+  orc_np_consts(h, k);
+  const long stride = dir == 0 ? 1 : dir == 1 ? nx : (long)nx * ny;
+  const int na = dir == 0 ? ny : nx, nb = dir == 2 ? ny : nz;
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < nb; ++b)
+    for (int a = 0; a < na; ++a) {
+      const size_t q = dir == 0 ? IDX(0, a, b, nx, ny) : dir == 1 ? IDX(a, 0, b, nx, ny) : IDX(a, b, 0, nx, ny);
+      for (int i = 0; i < n; ++i) t[q + (size_t)i * stride] = np_rhs(u + q, stride, i, n, ncl1, ncln, npaire, k);
+    }
+  if (dir == 0) orc_xthomas_12(t, ff, fs, fw, nx, ny, nz);
+  else if (dir == 1) orc_ythomas_12(t, ff, fs, fw, nx, ny, nz);
+  else orc_zthomas_12(t, ff, fs, fw, nx, ny, nz);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* FFT: a mixed-radix Cooley-Tukey DFT (any n, fast for small prime         */
 /* factors) used to restate 2decomp-fft's unnormalised PHYSICAL_IN_Z        */
 /* r2c/c2r (call sites src/poisson.f90:318,325,412).  Forward sign -1.      */

@@ -93,6 +93,15 @@ void orc_derx(const orc_ctx *c, double *t, const double *u, int op, int nx, int 
 void orc_dery(const orc_ctx *c, double *t, const double *u, int op, int nx, int ny, int nz);
 void orc_derz(const orc_ctx *c, double *t, const double *u, int op, int nx, int ny, int nz);
 
+/* ---- non-periodic first derivatives (der?_11/12/21/22) --------------------- */
+/* boundary constants alfai afi bfi alfa1 af1 bf1 cf1 alfa2 af2 alfan afn bfn cfn
+ * alfam afm for spacing h */
+void orc_np_consts(double h, double *k);
+int  orc_np_tables(int n, double h, int ncl1, int ncln, double *ff, double *fs, double *fw,
+                   double *ffp, double *fsp, double *fwp, double *rows, double *rowsp);
+int  orc_der_np(double *t, const double *u, int dir, int nx, int ny, int nz, double h, int ncl1,
+                int ncln, int npaire, const double *ff, const double *fs, const double *fw);
+
 /* ---- Poisson -------------------------------------------------------------- */
 void orc_fft_r2c_z(const orc_ctx *c, const double *rhs, double *cw1);
 void orc_fft_c2r_z(const orc_ctx *c, double *cw1, double *rhs);

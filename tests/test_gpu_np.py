@@ -1,0 +1,110 @@
+"""GPU parity of the non-periodic first derivatives (der?_11/12/21/22 with
+npaire, src/x3d_derive.f90:156-275, 343-499, 562-725) against the oracle
+(pinned by closed forms and exact cubics in tests/test_oracle_np.py), at
+1e-12 of the result's maximum, for every pair of boundary codes, both
+instances (derX npaire 0, derXp npaire 1), all three directions and chunk
+lengths 1-16; plus what such a grid refuses."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+BCS = [(1, 1), (1, 2), (2, 1), (2, 2)]
+# (nx, ny, nz, lengths): x chunks 8 / 2 / 1 / 12, y and z chunks 1-16
+GRIDS = [
+    (64, 48, 40, (1.0, 2.0, 3.0)),
+    (16, 512, 8, (2.0, 1.0, 0.5)),
+    (8, 24, 96, (1.5, 2.5, 1.0)),
+    (768, 16, 16, (2.0, 2.0, 2.0)),
+]
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g[:3])))
+@pytest.mark.parametrize("bc", BCS, ids=lambda b: f"{b[0]}{b[1]}")
+def test_derivatives_non_periodic(torch_cuda, grid, bc):
+    import torch
+    from x3div_amd import X3D
+    nx, ny, nz, L = grid
+    print(f"\n[np {nx}x{ny}x{nz} bc {bc}]", flush=True)
+    ctx = X3D(nx, ny, nz, *L, ncl1=(bc[0],) * 3, ncln=(bc[1],) * 3)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((nz, ny, nx))
+    ud = dev(u)
+    for axis, d in (("x", 0), ("y", 1), ("z", 2)):
+        for npaire, name in ((0, f"der{axis}"), (1, f"der{axis}p")):
+            t = ctx.field()
+            getattr(ctx, f"der{axis}")(t, ud, name)
+            torch.cuda.synchronize()
+            ref = oracle.der_np(u, axis, L[d], bc[0], bc[1], npaire)
+            e = rel(t.cpu().numpy(), ref)
+            assert e < 1e-12, (axis, name, e)
+    ctx.close()
+
+
+def test_mixed_directions_and_thomas(torch_cuda):
+    """x free-slip/Dirichlet, y periodic, z Dirichlet/free-slip in one grid;
+    the periodic direction keeps the periodic operators, and the Thomas entry
+    point of a non-periodic instance runs the _12 sweeps with its tables."""
+    import torch
+    from x3div_amd import X3D
+    nx, ny, nz = 32, 24, 16
+    ctx = X3D(nx, ny, nz, 1.0, 2.0, 3.0, ncl1=(1, 0, 2), ncln=(2, 0, 1))
+    orc = oracle.Oracle(nx, ny, nz, 1.0, 2.0, 3.0)
+    u = np.random.default_rng(4).standard_normal((nz, ny, nx))
+    t = ctx.field()
+    ctx.derx(t, dev(u), "derxp")
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy(), oracle.der_np(u, "x", 1.0, 1, 2, 1)) < 1e-12
+    ctx.dery(t, dev(u), "dery")
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy(), orc.apply("dery", u)) < 1e-12
+    ctx.derz(t, dev(u), "derz")
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy(), oracle.der_np(u, "z", 3.0, 2, 1, 0)) < 1e-12
+    # x3d_zthomas on derzp: the _12 sweeps with the npaire = 1 tables
+    tb = oracle.np_tables(nz, 3.0 / (nz - 1), 2, 1)
+    tt = dev(u)
+    ctx.zthomas(tt, "derzp")
+    torch.cuda.synchronize()
+    ref = u.copy()
+    oracle.lib(False).orc_zthomas_12(oracle._p(ref), oracle._p(tb["fp"]), oracle._p(tb["sp"]),
+                                     oracle._p(tb["wp"]), nx, ny, nz)
+    assert rel(tt.cpu().numpy(), ref) < 1e-12
+    # the instance's tables as x3d_operator_get returns them
+    f, s, w, p, alfa, npaire = ctx.operator("derzp")
+    assert np.array_equal(f, tb["fp"]) and np.array_equal(s, tb["sp"]) and np.array_equal(w, tb["wp"])
+    assert npaire == 1 and not p.any()
+    ctx.close()
+
+
+def test_non_periodic_refusals(torch_cuda):
+    from x3div_amd import X3D, X3DError
+    with pytest.raises(X3DError, match="ARG"):
+        X3D(16, 16, 16, ncl1=(0, 1, 0), ncln=(0, 0, 0))
+    ctx = X3D(16, 16, 16, ncl1=(1, 0, 0), ncln=(2, 0, 0))
+    a, b = ctx.field(), ctx.field()
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.apply("derxvp", a, b)
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.apply("intxpv", a, b)
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.poisson(a)
+    f = [ctx.field() for _ in range(10)]
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.step(*f, gdt=0.0)
+    with pytest.raises(X3DError, match="UNSUPPORTED"):
+        ctx.apply_f32("derx", ctx.field32(), ctx.field32())
+    # the periodic directions of the same grid still work
+    ctx.apply("deryvp", a, b)
+    ctx.close()

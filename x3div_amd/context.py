@@ -54,15 +54,20 @@ class X3D:
     """A periodic nx*ny*nz grid of lengths (xlx, yly, zlz) on the GPU."""
 
     def __init__(self, nx: int, ny: int, nz: int, xlx: float = 1.0, yly: float = 1.0,
-                 zlz: float = 1.0, stream=None):
+                 zlz: float = 1.0, stream=None, ncl1=(0, 0, 0), ncln=(0, 0, 0)):
+        """ncl1, ncln: boundary codes per direction as the reference's
+        nclx1/nclxn...: 0 periodic, 1 free-slip, 2 Dirichlet."""
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("x3div_amd needs a HIP device (no CPU fallback)")
         self.lib = _lib.load()
         self.nx, self.ny, self.nz = nx, ny, nz
         self.L = (xlx, yly, zlz)
+        self.ncl1, self.ncln = tuple(ncl1), tuple(ncln)
         h = C.c_void_p()
-        check("x3d_create", self.lib.x3d_create(nx, ny, nz, xlx, yly, zlz, C.byref(h)))
+        b1 = (C.c_int * 3)(*self.ncl1)
+        bn = (C.c_int * 3)(*self.ncln)
+        check("x3d_create_bc", self.lib.x3d_create_bc(nx, ny, nz, xlx, yly, zlz, b1, bn, C.byref(h)))
         self._h = h
         self.set_stream(stream if stream is not None else torch.cuda.current_stream())
 

@@ -115,6 +115,56 @@ static void direction_ops(x3d_ctx* c, int d, int n)
   op_init(c->op[X3D_OP_DERXX + d], f, s, w, sc.alsai, 0);
 }
 
+// A non-periodic direction (codes ncl1, ncln in {1, 2}; x3d_operator_1d_init
+// with first_deriv_imp_, src/x3d_operator_1d.f90:395-491): the two first
+// derivative instances take the non-periodic rows, prepared twice as the
+// reference does (derX: ff; derXp: ffp, which zeroes the free-slip corner
+// entries).  The staggered, interpolation and second-derivative instances
+// of that direction live on grids of n - 1 points in the reference and are
+// not built: they keep periodic tables of the right length and report
+// X3D_ERR_UNSUPPORTED.
+static void np_direction_ops(x3d_ctx* c, int d, int n, int ncl1, int ncln)
+{
+  direction_ops(c, d, n);
+  for (int op = 0; op < X3D_NOPS_ALL; ++op) {
+    const bool dd = op < 6 ? op / 2 == d : (op - 6) % 3 == d;
+    if (dd) c->op[op].bc_unsupported = true;
+  }
+  const double h = d == 0 ? c->dx : d == 1 ? c->dy : c->dz;
+  const double alfai = c->sc[d].alfai, alfa1 = 2.0, alfa2 = 1.0 / 4.0, alfan = 2.0, alfam = 1.0 / 4.0;
+  (void)h;
+  std::vector<double> ff(n, 0.0), fc(n, 0.0), fb(n, 0.0);
+  if (ncl1 == 1) {
+    ff[0] = alfai + alfai; ff[1] = alfai; fc[0] = 1.0; fc[1] = 1.0; fb[0] = alfai; fb[1] = alfai;
+  } else {
+    ff[0] = alfa1; ff[1] = alfa2; fc[0] = 1.0; fc[1] = 1.0; fb[0] = alfa2; fb[1] = alfai;
+  }
+  if (ncln == 1) {
+    ff[n - 3] = alfai; ff[n - 2] = alfai; ff[n - 1] = 0.0;
+    fc[n - 3] = 1.0; fc[n - 2] = 1.0; fc[n - 1] = 1.0;
+    fb[n - 3] = alfai; fb[n - 2] = alfai + alfai; fb[n - 1] = 0.0;
+  } else {
+    ff[n - 3] = alfai; ff[n - 2] = alfam; ff[n - 1] = 0.0;
+    fc[n - 3] = 1.0; fc[n - 2] = 1.0; fc[n - 1] = 1.0;
+    fb[n - 3] = alfam; fb[n - 2] = alfan; fb[n - 1] = 0.0;
+  }
+  for (int i = 2; i <= n - 4; ++i) { ff[i] = alfai; fc[i] = 1.0; fb[i] = alfai; }
+  std::vector<double> ffp = ff, s, w, sp, wp;
+  prepare(fb, fc, ff, s, w);
+  if (ncl1 == 1) ffp[0] = 0.0;
+  if (ncln == 1) fb[n - 2] = 0.0;
+  prepare(fb, fc, ffp, sp, wp);
+  for (int k = 0; k < 2; ++k) {
+    OpHost& o = c->op[X3D_OP_DERX + 2 * d + k];
+    op_init(o, k ? ffp : ff, k ? sp : s, k ? wp : w, alfai, k);
+    o.p.assign(n, 0.0);  // no periodic correction (x3doperator1d%init with ncl false)
+    o.p0 = o.p;
+    o.circulant = false;
+    o.np = true;
+    o.bc_unsupported = false;
+  }
+}
+
 static int upload_tables(x3d_ctx* c)
 {
   size_t len = 0;
@@ -413,10 +463,23 @@ static double* scratch_of(int dev)
 
 int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_ctx** out)
 {
+  return x3d_create_bc(nx, ny, nz, xlx, yly, zlz, nullptr, nullptr, out);
+}
+
+int x3d_create_bc(int nx, int ny, int nz, double xlx, double yly, double zlz, const int* ncl1, const int* ncln,
+                  x3d_ctx** out)
+{
   if (!out) return X3D_ERR_ARG;
   *out = nullptr;
   if (nx < 8 || ny < 8 || nz < 8) return X3D_ERR_SIZE;
   if (!(xlx > 0) || !(yly > 0) || !(zlz > 0)) return X3D_ERR_ARG;
+  int b1[3] = {0, 0, 0}, bn[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    if (ncl1) b1[d] = ncl1[d];
+    if (ncln) bn[d] = ncln[d];
+    // periodic at both ends or at neither (src/parameters.f90: nclx = nclx1 == 0 .and. nclxn == 0)
+    if (b1[d] < 0 || b1[d] > 2 || bn[d] < 0 || bn[d] > 2 || ((b1[d] == 0) != (bn[d] == 0))) return X3D_ERR_ARG;
+  }
   // the line kernels address a field with 32-bit byte offsets: FP32 fields
   // must stay below 4 GiB, and FP64 entry points need FP64 fields below it
   const unsigned long long npts = (unsigned long long)nx * ny * nz;
@@ -425,16 +488,22 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
   c->f64_lines = npts * sizeof(double) <= 0xFFFFFFFFull;
   c->nx = nx; c->ny = ny; c->nz = nz;
   c->xlx = xlx; c->yly = yly; c->zlz = zlz;
-  c->dx = xlx / (double)nx; c->dy = yly / (double)ny; c->dz = zlz / (double)nz;  // nXm = nX
+  // dX = L / nXm with nXm = nX periodic, nX - 1 otherwise (src/parameters.f90)
+  for (int d = 0; d < 3; ++d) { c->ncl1[d] = b1[d]; c->ncln[d] = bn[d]; }
+  c->dx = xlx / (double)(b1[0] ? nx - 1 : nx);
+  c->dy = yly / (double)(b1[1] ? ny - 1 : ny);
+  c->dz = zlz / (double)(b1[2] ? nz - 1 : nz);
   int rc = X3D_OK;
   if (hipGetDevice(&c->device) != hipSuccess) { delete c; return X3D_ERR_HIP; }
   if ((rc = scratch_setup(c->device)) != X3D_OK) { delete c; return rc; }
   scheme(c->sc[0], c->dx);
   scheme(c->sc[1], c->dy);
   scheme(c->sc[2], c->dz);
-  direction_ops(c, 0, nx);
-  direction_ops(c, 1, ny);
-  direction_ops(c, 2, nz);
+  const int ns[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) {
+    if (b1[d]) np_direction_ops(c, d, ns[d], b1[d], bn[d]);
+    else direction_ops(c, d, ns[d]);
+  }
   if (const char* xr = std::getenv("X3D_XCD_REMAP")) c->xcd_remap = std::atoi(xr) != 0;
   // tiling variant of the fused y/z passes (X3D_FUSED_TILING=0|1, default 0)
   const char* tv = std::getenv("X3D_FUSED_TILING");
@@ -569,7 +638,7 @@ int x3d_operator_set(x3d_ctx* c, int op, const double* f, const double* s, const
   if (own && o.circulant) return X3D_OK;
   o.f.assign(f, f + n); o.s.assign(s, s + n); o.w.assign(w, w + n); o.p.assign(p, p + n);
   o.alfa = alfa;
-  o.circulant = own;  // caller tables: solved with the table sweeps
+  o.circulant = own && !o.np;  // caller tables: solved with the table sweeps
   X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
   // a step captured earlier ran the circulant kernels on the old device
   // tables, which upload_tables replaces: it must not be replayed
@@ -773,6 +842,7 @@ int x3d_transpose_y_to_x(x3d_ctx* c, const void* in, void* out, size_t eb, int n
 int x3d_poisson(x3d_ctx* c, double* rhs)
 {
   if (!c || !rhs) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   return poisson_solve(c, rhs);
 }
 
@@ -860,6 +930,7 @@ int x3d_momentum_rhs(x3d_ctx* c, double* dux, double* duy, double* duz, const do
                      const double* uy, const double* uz)
 {
   if (!c || !dux || !duy || !duz || !ux || !uy || !uz) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   if (!fused_ok(c)) return ref_momentum_rhs(c, dux, duy, duz, ux, uy, uz);
   // the fused passes do not write u when euler = 0
   return fused_momentum<double>(c, 0.0, 0, const_cast<double*>(ux), const_cast<double*>(uy),
@@ -879,6 +950,7 @@ int x3d_int_time(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, con
 int x3d_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, const double* uz)
 {
   if (!c || !pp3 || !ux || !uy || !uz) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   if (!fused_ok(c)) return ref_divergence(c, pp3, ux, uy, uz);
   return fused_divergence<double>(c, pp3, ux, uy, uz);
 }
@@ -886,6 +958,7 @@ int x3d_divergence(x3d_ctx* c, double* pp3, const double* ux, const double* uy, 
 int x3d_gradp(x3d_ctx* c, double* px, double* py, double* pz, const double* pp3)
 {
   if (!c || !px || !py || !pz || !pp3) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   if (!fused_ok(c)) return ref_gradp(c, px, py, pz, pp3);
   return fused_gradp<double>(c, px, py, pz, pp3, nullptr, nullptr, nullptr, 0);
 }
@@ -913,6 +986,7 @@ int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double*
              double* duz, double* pp3, double* px, double* py, double* pz)
 {
   if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   if (!fused_ok(c)) {
     X3D_TRY(x3d_momentum_rhs(c, dux, duy, duz, ux, uy, uz));
     X3D_TRY(x3d_int_time(c, gdt, ux, uy, uz, dux, duy, duz));
@@ -977,6 +1051,7 @@ int x3d_apply_f32(x3d_ctx* c, int op, float* out, const float* in)
 int x3d_poisson_f32(x3d_ctx* c, float* rhs)
 {
   if (!c || !rhs) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   return poisson_solve_f32(c, rhs);
 }
 
@@ -989,6 +1064,7 @@ int x3d_step_f32(x3d_ctx* c, double gdt, float* ux, float* uy, float* uz, float*
                  float* pp3, float* px, float* py, float* pz)
 {
   if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   if (!fp32_ok(c)) return X3D_ERR_UNSUPPORTED;
   X3D_TRY(fused_momentum<float>(c, (float)gdt, 1, ux, uy, uz, dux, duy, duz));
   X3D_TRY(fused_divergence<float>(c, pp3, ux, uy, uz));
@@ -1150,6 +1226,7 @@ int x3d_step_noalloc(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz,
                      double* duz, double* pp3, double* px, double* py, double* pz)
 {
   if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
+  if (!all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   const size_t N = (size_t)c->nx * c->ny * c->nz;
   if (N > (size_t)X3D_NOALLOC_MAX_POINTS) return X3D_ERR_SIZE;
   double* base = scratch_of(c->device);

@@ -96,7 +96,7 @@ int x3d_step_bfp(x3d_ctx* c, double gdt, int32_t* ux, int32_t* uy, int32_t* uz, 
 {
   if (!c || !ux || !uy || !uz || !dux || !duy || !duz || !pp3 || !px || !py || !pz) return X3D_ERR_ARG;
   X3D_TRY(bfp_check(c));
-  if (!fused_ok(c)) return X3D_ERR_UNSUPPORTED;
+  if (!fused_ok(c) || !all_periodic(c)) return X3D_ERR_UNSUPPORTED;
   auto B = [](int32_t* p) { return reinterpret_cast<bfp32*>(p); };
   X3D_TRY(fused_momentum<bfp32>(c, gdt, 1, B(ux), B(uy), B(uz), B(dux), B(duy), B(duz)));
   X3D_TRY(fused_divergence<bfp32>(c, B(pp3), B(ux), B(uy), B(uz)));

@@ -94,6 +94,70 @@ void k_line(const S* in, S* out, OpDev<typename Store<S>::T> op, Coef<typename S
   }
 }
 
+// Right-hand side of the non-periodic first derivative at point j of n from
+// a window whose entry q is point j (der?_ij, src/x3d_derive.f90:172-210):
+// the interior sixth-order stencil, and at j = 0, 1, n-2, n-1 the free-slip
+// rows (npaire: even or odd extension) or the one-sided Dirichlet closures.
+// Every boundary row reads points inside [j - 2, j + 2], so the window holds
+// them whatever the chunk.
+__device__ __forceinline__ double np_rhs(const double* w, int q, int j, int n, const NpRows& r)
+{
+  if (j >= 2 && j <= n - 3) return r.afi * (w[q + 1] - w[q - 1]) + r.bfi * (w[q + 2] - w[q - 2]);
+  if (j == 0) {
+    if (r.ncl1 == 1) return r.npaire == 1 ? 0.0 : r.afi * (w[q + 1] + w[q + 1]) + r.bfi * (w[q + 2] + w[q + 2]);
+    return r.af1 * w[q] + r.bf1 * w[q + 1] + r.cf1 * w[q + 2];
+  }
+  if (j == 1) {
+    if (r.ncl1 == 1)
+      return r.npaire == 1 ? r.afi * (w[q + 1] - w[q - 1]) + r.bfi * (w[q + 2] - w[q])
+                           : r.afi * (w[q + 1] - w[q - 1]) + r.bfi * (w[q + 2] + w[q]);
+    return r.af2 * (w[q + 1] - w[q - 1]);
+  }
+  if (j == n - 2) {
+    if (r.ncln == 1)
+      return r.npaire == 1 ? r.afi * (w[q + 1] - w[q - 1]) + r.bfi * (w[q] - w[q - 2])
+                           : r.afi * (w[q + 1] - w[q - 1]) + r.bfi * ((-w[q]) - w[q - 2]);
+    return r.afm * (w[q + 1] - w[q - 1]);
+  }
+  if (r.ncln == 1) return r.npaire == 1 ? 0.0 : r.afi * ((-w[q - 1]) - w[q - 1]) + r.bfi * ((-w[q - 2]) - w[q - 2]);
+  return -r.afn * w[q] - r.bfn * w[q - 1] - r.cfn * w[q - 2];
+}
+
+// One non-periodic first derivative along x, y or z: the right-hand side,
+// then the reference's Thomas sweeps with the instance's tables and no
+// periodic correction (xthomas_12, src/thomas.f90:61-84).  The windows wrap
+// at the line ends like the periodic kernels'; the boundary rows never read
+// the wrapped points.
+template <int M, bool LF>
+__global__ __launch_bounds__(LF ? 1024 : 256) void k_der_np(const double* in, double* out, OpDev<double> op, NpRows r,
+                                                            Geom geo, Tiling tl)
+{
+  constexpr int H = 2;
+  extern __shared__ unsigned char smem_raw[];
+  double* lds = reinterpret_cast<double*>(smem_raw);
+  const LineThread t = line_thread<double>(geo, tl, M, blockIdx.x);
+  const int n = geo.n;
+  double v[M];
+  if constexpr (LF) {
+    double win[M + 2 * H];
+    load_window<double, M, H, H, true>(in, t, n, win);
+#pragma unroll
+    for (int k = 0; k < M; ++k) v[k] = np_rhs(win, k + H, t.j0 + k, n, r);
+    solve_table<double, M, true>(v, t, op, lds, n, false);
+    store_chunk<double, M>(out, t, v);
+  } else {
+    const XBlock<double, M> xb = x_block<double, M>(geo, t, lds);
+    {
+      double win[M + 2 * H];
+      x_load_window<double, M, H, H>(in, xb, t, win);
+#pragma unroll
+      for (int k = 0; k < M; ++k) v[k] = np_rhs(win, k + H, t.j0 + k, n, r);
+    }
+    solve_table<double, M, false>(v, t, op, lds, n, false);
+    x_store<double, M>(out, xb, v);
+  }
+}
+
 // Points per chunk the kernels are instantiated for.
 static const int kMs[] = {1, 2, 3, 4, 5, 6, 8, 12, 16};
 
@@ -277,6 +341,13 @@ static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out, bool periodic_fix)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS_ALL || !in || !out) return X3D_ERR_ARG;
+  if (c->op[op].bc_unsupported) return X3D_ERR_UNSUPPORTED;
+  if (c->op[op].np) {
+    // non-periodic first derivatives: der?_ij; Thomas alone: the _12 sweeps
+    if (kind == K_D) return launch_line_np(c, dir, op, in, out);
+    if (kind != K_NONE) return X3D_ERR_UNSUPPORTED;
+    periodic_fix = false;
+  }
   if (!c->f64_lines) return X3D_ERR_UNSUPPORTED;
   const LinePlan& P = c->plan[dir];
   const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
@@ -355,6 +426,51 @@ int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, floa
   const OpDev<float> od{nullptr, nullptr, nullptr, nullptr, (float)c->op[op].alfa};
   const hipError_t e = launch_any<float, true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   return e == hipSuccess ? X3D_OK : hip_status(e);
+}
+
+NpRows np_rows(const x3d_ctx* c, int dir, int op)
+{
+  // first_deriv_exp_, ifirstder = 4 (src/x3d_operator_1d.f90:350-391)
+  const double h = dir == 0 ? c->dx : dir == 1 ? c->dy : c->dz;
+  NpRows r;
+  r.ncl1 = c->ncl1[dir]; r.ncln = c->ncln[dir]; r.npaire = c->op[op].npaire;
+  r.afi = c->sc[dir].afi; r.bfi = c->sc[dir].bfi;
+  r.af1 = -(5.0 / 2.0) / h; r.bf1 = 2.0 / h; r.cf1 = 0.5 / h;
+  r.af2 = (3.0 / 4.0) / h;
+  r.afn = -(5.0 / 2.0) / h; r.bfn = 2.0 / h; r.cfn = 0.5 / h;
+  r.afm = (3.0 / 4.0) / h;
+  return r;
+}
+
+int launch_line_np(x3d_ctx* c, int dir, int op, const double* in, double* out)
+{
+  if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS_ALL || !in || !out) return X3D_ERR_ARG;
+  if (!c->f64_lines) return X3D_ERR_UNSUPPORTED;
+  const LinePlan& P = c->plan[dir];
+  const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
+  if (c->op[op].n != n || P.n != n) return X3D_ERR_SIZE;
+  if (!c->op[op].np || P.M == 0) return X3D_ERR_UNSUPPORTED;
+  Geom geo;
+  Tiling tl;
+  dim3 grid, block;
+  size_t smem;
+  launch_dims(c, dir, &geo, &tl, &grid, &block, &smem, sizeof(double));
+  tl.xcd = 0;
+  const double* tb = c->tab64 + c->off[op];
+  const OpDev<double> od{tb, tb + n, tb + 2 * n, tb + 3 * n, c->op[op].alfa};
+  const NpRows r = np_rows(c, dir, op);
+#define X3D_NP(MM)                                                                                     \
+  case MM:                                                                                             \
+    if (P.line_fast) hipLaunchKernelGGL((k_der_np<MM, true>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
+    else hipLaunchKernelGGL((k_der_np<MM, false>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
+    break;
+  switch (P.M) {
+    X3D_NP(1) X3D_NP(2) X3D_NP(3) X3D_NP(4) X3D_NP(5) X3D_NP(6) X3D_NP(8) X3D_NP(12) X3D_NP(16)
+    default: return X3D_ERR_UNSUPPORTED;
+  }
+#undef X3D_NP
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
 }
 
 // BFP32 form of one operator: BFP32 fields, the circulant solver in FP64 on

@@ -36,6 +36,8 @@ struct OpHost {
   int n = 0, npaire = 0;
   double alfa = 0.0;
   bool circulant = false;  // tables are the library's periodic [alfa, 1, alfa] ones
+  bool np = false;         // non-periodic first derivative (der?_ij): boundary rows, no periodic correction
+  bool bc_unsupported = false;  // an operator of a non-periodic direction this library does not build
   std::vector<double> f, s, w, p;
   std::vector<double> f0, s0, w0, p0;  // the library's own tables, kept to recognise them
   double alfa0 = 0.0;
@@ -72,6 +74,7 @@ struct x3d_ctx {
   bool f64_lines = true;  // FP64 fields below 4 GiB (32-bit offsets of the line kernels)
   hipStream_t stream = nullptr;
   x3d::Scheme sc[3];
+  int ncl1[3] = {0, 0, 0}, ncln[3] = {0, 0, 0};  // boundary codes per direction (0 periodic)
   x3d::OpHost op[X3D_NOPS_ALL];
   // device operator tables: for op o, f,s,w,p at tab64 + off[o] + {0,1,2,3}*n
   double* tab64 = nullptr;
@@ -174,6 +177,15 @@ struct bfp32 {
   int32_t w;
 };
 int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp32* out);
+// Boundary rows of the non-periodic first derivative der?_ij
+// (src/x3d_derive.f90:156-217): codes 1 free-slip, 2 Dirichlet at each end,
+// the instance's npaire, and first_deriv_exp_'s constants for the spacing.
+struct NpRows {
+  int ncl1, ncln, npaire;
+  double afi, bfi, af1, bf1, cf1, af2, afm, afn, bfn, cfn;
+};
+NpRows np_rows(const x3d_ctx* c, int dir, int op);
+int launch_line_np(x3d_ctx* c, int dir, int op, const double* in, double* out);
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
                 bool periodic_fix);
 // The Thomas sweeps alone (in place) with device tables f, s, w, p at
@@ -199,6 +211,15 @@ inline int mark_pass(x3d_ctx* c, int i)
 {
   if (c->timing && hipEventRecord(c->pass_ev[i], c->stream) != hipSuccess) return X3D_ERR_HIP;
   return X3D_OK;
+}
+
+// Triply periodic: the reference's Poisson solver stops otherwise
+// (src/poisson.f90:104-108), and so do the step and its Poisson entry points.
+inline bool all_periodic(const x3d_ctx* c)
+{
+  for (int d = 0; d < 3; ++d)
+    if (c->ncl1[d] || c->ncln[d]) return false;
+  return true;
 }
 
 // Fused step passes (all operators must carry the library's periodic tables).

@@ -26,6 +26,13 @@ module x3d_hip
       real(c_double), value :: xlx, yly, zlz
       type(c_ptr) :: ctx
     end function
+    integer(c_int) function x3d_create_bc(nx, ny, nz, xlx, yly, zlz, ncl1, ncln, ctx) bind(C, name="x3d_create_bc")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nx, ny, nz
+      real(c_double), value :: xlx, yly, zlz
+      integer(c_int), intent(in) :: ncl1(3), ncln(3)
+      type(c_ptr) :: ctx
+    end function
     integer(c_int) function x3d_destroy(ctx) bind(C, name="x3d_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -330,7 +330,8 @@ contains
 end module thomas
 
 !! Derivative procedure pointers (src/x3d_derive.f90:18-109), periodic derx_00,
-!! dery_00, derz_00 (:111-152, :279-339, :503-558).
+!! dery_00, derz_00 (:111-152, :279-339, :503-558) and the non-periodic
+!! der?_11/12/21/22 (:219-275, :386-499, :608-725).
 module x3d_derive
   use, intrinsic :: iso_c_binding
   use decomp_2d_constants, only : mytype
@@ -342,6 +343,8 @@ module x3d_derive
   public :: derxx, deryy, derzz
   public :: x3d_derive_init, x3d_derive_finalize, derx_00, dery_00, derz_00
   public :: derxx_00, deryy_00, derzz_00
+  public :: derx_11, derx_12, derx_21, derx_22, dery_11, dery_12, dery_21, dery_22
+  public :: derz_11, derz_12, derz_21, derz_22
 
   abstract interface
     subroutine DERIVATIVE_X(t, u, x3dop, nx, ny, nz)
@@ -382,11 +385,25 @@ module x3d_derive
 
 contains
 
-  !> x3d_derive_init (src/x3d_derive.f90:66-93), periodic BCs.
+  !> x3d_derive_init (src/x3d_derive.f90:66-93): the pointers chosen by the
+  !> boundary codes of module param.
   subroutine x3d_derive_init()
-    derx => derx_00
-    dery => dery_00
-    derz => derz_00
+    use param, only : nclx1, nclxn, ncly1, nclyn, nclz1, nclzn
+    if (nclx1 == 0 .and. nclxn == 0) derx => derx_00
+    if (nclx1 == 1 .and. nclxn == 1) derx => derx_11
+    if (nclx1 == 1 .and. nclxn == 2) derx => derx_12
+    if (nclx1 == 2 .and. nclxn == 1) derx => derx_21
+    if (nclx1 == 2 .and. nclxn == 2) derx => derx_22
+    if (ncly1 == 0 .and. nclyn == 0) dery => dery_00
+    if (ncly1 == 1 .and. nclyn == 1) dery => dery_11
+    if (ncly1 == 1 .and. nclyn == 2) dery => dery_12
+    if (ncly1 == 2 .and. nclyn == 1) dery => dery_21
+    if (ncly1 == 2 .and. nclyn == 2) dery => dery_22
+    if (nclz1 == 0 .and. nclzn == 0) derz => derz_00
+    if (nclz1 == 1 .and. nclzn == 1) derz => derz_11
+    if (nclz1 == 1 .and. nclzn == 2) derz => derz_12
+    if (nclz1 == 2 .and. nclzn == 1) derz => derz_21
+    if (nclz1 == 2 .and. nclzn == 2) derz => derz_22
     ! second derivatives: the reference leaves these pointers unassociated
     derxx => derxx_00
     deryy => deryy_00
@@ -447,5 +464,99 @@ contains
     type(x3doperator1d), intent(in) :: x3dop
     call x3d_check('derz', x3d_derz(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), nx, ny, nz))
   end subroutine derz_00
+
+  !> der?_11/12/21/22: the context was made with these codes
+  !> (x3d_create_bc), so the library's derivative entry points run the
+  !> non-periodic rows; the routines keep the reference's names.
+  subroutine derx_11(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derx_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derx_11
+  subroutine derx_12(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derx_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derx_12
+  subroutine derx_21(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derx_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derx_21
+  subroutine derx_22(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derx_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derx_22
+
+  subroutine dery_11(t, u, x3dop, pp, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    real(mytype), intent(in), dimension(ny) :: pp
+    type(x3doperator1d), intent(in) :: x3dop
+    call dery_00(t, u, x3dop, pp, nx, ny, nz)
+  end subroutine dery_11
+  subroutine dery_12(t, u, x3dop, pp, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    real(mytype), intent(in), dimension(ny) :: pp
+    type(x3doperator1d), intent(in) :: x3dop
+    call dery_00(t, u, x3dop, pp, nx, ny, nz)
+  end subroutine dery_12
+  subroutine dery_21(t, u, x3dop, pp, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    real(mytype), intent(in), dimension(ny) :: pp
+    type(x3doperator1d), intent(in) :: x3dop
+    call dery_00(t, u, x3dop, pp, nx, ny, nz)
+  end subroutine dery_21
+  subroutine dery_22(t, u, x3dop, pp, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    real(mytype), intent(in), dimension(ny) :: pp
+    type(x3doperator1d), intent(in) :: x3dop
+    call dery_00(t, u, x3dop, pp, nx, ny, nz)
+  end subroutine dery_22
+
+  subroutine derz_11(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derz_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derz_11
+  subroutine derz_12(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derz_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derz_12
+  subroutine derz_21(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derz_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derz_21
+  subroutine derz_22(t, u, x3dop, nx, ny, nz)
+    integer, intent(in) :: nx, ny, nz
+    real(mytype), intent(out), dimension(nx, ny, nz), target :: t
+    real(mytype), intent(in), dimension(nx, ny, nz), target :: u
+    type(x3doperator1d), intent(in) :: x3dop
+    call derz_00(t, u, x3dop, nx, ny, nz)
+  end subroutine derz_22
 
 end module x3d_derive

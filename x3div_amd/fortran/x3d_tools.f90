@@ -18,9 +18,9 @@ subroutine init_xcompact3d(ndt_max)
   use decomp_2d_poisson, only : decomp_2d_poisson_init
   use x3d_operator_1d, only : x3d_operator_1d_init
   use x3d_derive, only : x3d_derive_init
-  use x3d_hip, only : x3d_ctx_h, x3d_check, x3d_create
+  use x3d_hip, only : x3d_ctx_h, x3d_check, x3d_create_bc
   use parameters, only : parameter, listing
-  use param, only : xlx, yly, zlz
+  use param, only : xlx, yly, zlz, nclx1, nclxn, ncly1, nclyn, nclz1, nclzn
   use var, only : var_init
   use variables, only : nx, ny, nz, nxm, nym, nzm, p_row, p_col, test_mode
   implicit none
@@ -68,8 +68,11 @@ subroutine init_xcompact3d(ndt_max)
   call decomp_info_init(nxm, nym, nz, ph3)
 
   ! the library context: operator tables, line plans, FFT plans on this GPU
-  call x3d_check('x3d_create', x3d_create(int(nx, c_int), int(ny, c_int), int(nz, c_int), &
-                                          real(xlx, c_double), real(yly, c_double), real(zlz, c_double), x3d_ctx_h))
+  ! with parameter()'s boundary codes (all periodic in this driver)
+  call x3d_check('x3d_create', x3d_create_bc(int(nx, c_int), int(ny, c_int), int(nz, c_int), &
+                                             real(xlx, c_double), real(yly, c_double), real(zlz, c_double), &
+                                             int([nclx1, ncly1, nclz1], c_int), int([nclxn, nclyn, nclzn], c_int), &
+                                             x3d_ctx_h))
   call var_init()
   call x3d_operator_1d_init()
   call x3d_derive_init()
