@@ -54,7 +54,7 @@ def parse():
 def profile_kernels(n: int, reps: int = 3):
     """Launches for hardware-counter runs: a device-to-device copy of one n^3 FP64
     field (known bytes: 8 n^3 read + 8 n^3 written, the counter calibration)
-    and whole steps of both forms, whose line passes are then read per kernel."""
+    and whole steps of the three forms, whose line passes are then read per kernel."""
     import numpy as np
     import torch
 
@@ -71,9 +71,15 @@ def profile_kernels(n: int, reps: int = 3):
     f32 = [t.float() for t in f[:3]] + [ctx.field32() for _ in range(7)]
     for _ in range(reps):
         ctx.step_f32(*f32, gdt=0.0)
+    del f32
+    fb = [ctx.bfp_empty() for _ in range(10)]
+    for i in range(3):
+        ctx.bfp_encode(fb[i], f[i])
+    for _ in range(reps):
+        ctx.step_bfp(*fb, gdt=0.0)
     torch.cuda.synchronize()
     ctx.close()
-    print(f"profile-kernels: {n}^3, {reps} copies of {8 * n ** 3} B, {reps} FP64 steps and {reps} FP32 steps",
+    print(f"profile-kernels: {n}^3, {reps} copies of {8 * n ** 3} B, {reps} FP64, {reps} FP32 and {reps} BFP32 steps",
           flush=True)
 
 

@@ -489,6 +489,7 @@ int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp3
   dim3 grid, block;
   size_t smem;
   launch_dims_plan(c, dir, P, &geo, &tl, &grid, &block, &smem, sizeof(double));
+  if (dir == 2) tl.xcd = c->xcd_remap;  // 64 B rows: both halves of a 128 B line on one XCD
   const Coef<double> k = stencil_coef(c->sc[dir], kind);
   const double ig = 1.0 / circ_gamma(c->op[op].alfa);
   const Coef<double> cf{k.a * ig, k.b * ig, k.c * ig, k.d * ig};

@@ -431,6 +431,11 @@ static DirSet<S> dirset(const x3d_ctx* c, int dir)
   DirSet<S> s;
   const LinePlan& P = fused_plan<S>(c, dir);
   launch_dims_plan(c, dir, P, &s.geo, &s.tl, &s.grid, &s.block, &s.smem, sizeof(T));
+  // BFP32 z-rows of 16 lines are 64 B, half a 128 B L2 line whose other half
+  // belongs to the next line block: blocks regrouped per XCD fetch it into
+  // one L2 once (the dispatcher's round robin puts the two on different XCDs)
+  if constexpr (is_bfp<S>)
+    if (dir == 2) s.tl.xcd = c->xcd_remap;
   s.M = P.M;
   // keep slots of transeq: the u_d window (M + 4) and, along z, the u_c chunk
   s.smem_tq = s.smem;
