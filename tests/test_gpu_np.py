@@ -12,12 +12,17 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 BCS = [(1, 1), (1, 2), (2, 1), (2, 2)]
-# (nx, ny, nz, lengths): x chunks 8 / 2 / 1 / 12, y and z chunks 1-16
+# (nx, ny, nz, lengths).  Non-periodic lines have any length: the reference's
+# grids are 2^k + 1 points (65, 129, 257, 513 here), padded to C chunks of M
+# points (513 -> 64 x 9, 257 -> 32 x 9, 129 -> 16 x 9, 65 -> 64 x 2 ...),
+# and lengths that factor exactly (64, 48, 96, 768) as well.
 GRIDS = [
     (64, 48, 40, (1.0, 2.0, 3.0)),
-    (16, 512, 8, (2.0, 1.0, 0.5)),
-    (8, 24, 96, (1.5, 2.5, 1.0)),
-    (768, 16, 16, (2.0, 2.0, 2.0)),
+    (65, 33, 17, (1.0, 2.0, 3.0)),
+    (513, 9, 12, (2.0, 1.0, 0.5)),
+    (9, 257, 24, (1.5, 2.5, 1.0)),
+    (16, 10, 129, (2.0, 1.0, 2.0)),
+    (768, 16, 96, (2.0, 2.0, 2.0)),
 ]
 
 
@@ -58,7 +63,7 @@ def test_mixed_directions_and_thomas(torch_cuda):
     point of a non-periodic instance runs the _12 sweeps with its tables."""
     import torch
     from x3div_amd import X3D
-    nx, ny, nz = 32, 24, 16
+    nx, ny, nz = 33, 24, 17
     ctx = X3D(nx, ny, nz, 1.0, 2.0, 3.0, ncl1=(1, 0, 2), ncln=(2, 0, 1))
     orc = oracle.Oracle(nx, ny, nz, 1.0, 2.0, 3.0)
     u = np.random.default_rng(4).standard_normal((nz, ny, nx))

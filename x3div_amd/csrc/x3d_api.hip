@@ -128,7 +128,10 @@ static void np_direction_ops(x3d_ctx* c, int d, int n, int ncl1, int ncln)
   direction_ops(c, d, n);
   for (int op = 0; op < X3D_NOPS_ALL; ++op) {
     const bool dd = op < 6 ? op / 2 == d : (op - 6) % 3 == d;
-    if (dd) c->op[op].bc_unsupported = true;
+    if (dd) {
+      c->op[op].bc_unsupported = true;
+      c->op[op].circulant = false;  // no form may take them for periodic
+    }
   }
   const double h = d == 0 ? c->dx : d == 1 ? c->dy : c->dz;
   const double alfai = c->sc[d].alfai, alfa1 = 2.0, alfa2 = 1.0 / 4.0, alfan = 2.0, alfam = 1.0 / 4.0;
@@ -163,6 +166,32 @@ static void np_direction_ops(x3d_ctx* c, int d, int n, int ncl1, int ncln)
     o.np = true;
     o.bc_unsupported = false;
   }
+}
+
+// The non-periodic instances' tables padded to their direction's C*M points
+// with identity rows (f = s = 0, w = 1), in one device array.
+static int np_upload(x3d_ctx* c)
+{
+  std::vector<double> host;
+  for (int op = 0; op < X3D_NOPS_ALL; ++op) {
+    OpHost& o = c->op[op];
+    if (!o.np) continue;
+    const int d = op / 2;  // the derivative instances: 2d, 2d + 1
+    const size_t np = (size_t)c->npplan[d].C * c->npplan[d].M;
+    o.npoff = host.size();
+    host.resize(host.size() + 3 * np, 0.0);
+    double* f = host.data() + o.npoff;
+    for (size_t j = 0; j < np; ++j) {
+      const bool real = j < (size_t)o.n;
+      f[j] = real ? o.f[j] : 0.0;
+      f[np + j] = real ? o.s[j] : 0.0;
+      f[2 * np + j] = real ? o.w[j] : 1.0;
+    }
+  }
+  if (host.empty()) return X3D_OK;
+  if (!c->nptab) X3D_CHECK_HIP(hipMalloc(&c->nptab, host.size() * sizeof(double)));
+  X3D_CHECK_HIP(hipMemcpy(c->nptab, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+  return X3D_OK;
 }
 
 static int upload_tables(x3d_ctx* c)
@@ -513,11 +542,26 @@ int x3d_create_bc(int nx, int ny, int nz, double xlx, double yly, double zlz, co
   if (line_plan_fused32(nx, false, &c->fplan32[0]) || line_plan_fused32(ny, true, &c->fplan32[1]) ||
       line_plan_fused32(nz, true, &c->fplan32[2]))
     c->fplan32[0].M = c->fplan32[1].M = c->fplan32[2].M = 0;
-  if ((rc = line_plan(nx, false, &c->plan[0])) || (rc = line_plan(ny, true, &c->plan[1])) ||
-      (rc = line_plan(nz, true, &c->plan[2])) || (rc = line_plan_fused(nx, false, &c->fplan[0], variant)) ||
-      (rc = line_plan_fused(ny, true, &c->fplan[1], variant)) ||
-      (rc = line_plan_fused(nz, true, &c->fplan[2], variant)) ||
-      (rc = upload_tables(c)) || (rc = poisson_setup(c)) ||
+  // a non-periodic direction runs its own kernels on its own plan (lines of
+  // any length); the periodic plans there may not exist, and are not used
+  for (int d = 0; d < 3; ++d) {
+    if (!b1[d]) continue;
+    if ((rc = np_plan(ns[d], d != 0, &c->npplan[d]))) { x3d_destroy(c); return rc; }
+    if (line_plan(ns[d], d != 0, &c->plan[d])) c->plan[d] = LinePlan{};
+    if (line_plan_fused(ns[d], d != 0, &c->fplan[d], variant)) c->fplan[d] = LinePlan{};
+  }
+  auto plans = [&]() -> int {
+    int r = X3D_OK;
+    if (!b1[0] && ((r = line_plan(nx, false, &c->plan[0])) || (r = line_plan_fused(nx, false, &c->fplan[0], variant))))
+      return r;
+    if (!b1[1] && ((r = line_plan(ny, true, &c->plan[1])) || (r = line_plan_fused(ny, true, &c->fplan[1], variant))))
+      return r;
+    if (!b1[2] && ((r = line_plan(nz, true, &c->plan[2])) || (r = line_plan_fused(nz, true, &c->fplan[2], variant))))
+      return r;
+    return r;
+  };
+  if ((rc = plans()) || (rc = upload_tables(c)) || (rc = np_upload(c)) ||
+      (all_periodic(c) && (rc = poisson_setup(c))) ||
       (rc = hip_status_or_ok(hipMalloc(&c->usertab, 4 * sizeof(double) * (size_t)std::max(nx, std::max(ny, nz)))))) {
     x3d_destroy(c);
     return rc;
@@ -543,6 +587,7 @@ int x3d_destroy(x3d_ctx* c)
   for (void* t : c->circ_tab)
     if (t) (void)hipFree(t);
   if (c->usertab) (void)hipFree(c->usertab);
+  if (c->nptab) (void)hipFree(c->nptab);
   if (c->cap_stream) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(c->cap_stream, &g);
@@ -643,7 +688,8 @@ int x3d_operator_set(x3d_ctx* c, int op, const double* f, const double* s, const
   // a step captured earlier ran the circulant kernels on the old device
   // tables, which upload_tables replaces: it must not be replayed
   graph_release(c);
-  return upload_tables(c);
+  int rc = upload_tables(c);
+  return rc ? rc : (o.np ? np_upload(c) : X3D_OK);
 }
 
 int x3d_scheme_consts(const x3d_ctx* c, double* out)

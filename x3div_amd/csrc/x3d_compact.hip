@@ -123,38 +123,51 @@ __device__ __forceinline__ double np_rhs(const double* w, int q, int j, int n, c
   return -r.afn * w[q] - r.bfn * w[q - 1] - r.cfn * w[q - 2];
 }
 
-// One non-periodic first derivative along x, y or z: the right-hand side,
-// then the reference's Thomas sweeps with the instance's tables and no
-// periodic correction (xthomas_12, src/thomas.f90:61-84).  The windows wrap
-// at the line ends like the periodic kernels'; the boundary rows never read
-// the wrapped points.
-template <int M, bool LF>
+// One non-periodic first derivative along x, y or z (RHS), or the Thomas
+// sweeps alone: the right-hand side, then the reference's sweeps with the
+// instance's tables and no periodic correction (xthomas_12,
+// src/thomas.f90:61-84).  Lines have any length n: they are cut into C
+// chunks of M points with C M >= n, and the points past n are identity rows
+// of the padded tables (f = s = 0, w = 1) with a zero right-hand side, which
+// the sweeps carry through without touching the real points (f(n-1) = 0).
+// Every load is clamped into the line (the clamped values feed no real row)
+// and only real points are stored.  The Thomas form reads its own chunk
+// alone, so it may run in place.
+template <int M, bool LF, bool RHS>
 __global__ __launch_bounds__(LF ? 1024 : 256) void k_der_np(const double* in, double* out, OpDev<double> op, NpRows r,
                                                             Geom geo, Tiling tl)
 {
-  constexpr int H = 2;
+  constexpr int H = RHS ? 2 : 0;
   extern __shared__ unsigned char smem_raw[];
   double* lds = reinterpret_cast<double*>(smem_raw);
   const LineThread t = line_thread<double>(geo, tl, M, blockIdx.x);
-  const int n = geo.n;
+  const int n = r.n;
   double v[M];
-  if constexpr (LF) {
+  {
     double win[M + 2 * H];
-    load_window<double, M, H, H, true>(in, t, n, win);
 #pragma unroll
-    for (int k = 0; k < M; ++k) v[k] = np_rhs(win, k + H, t.j0 + k, n, r);
-    solve_table<double, M, true>(v, t, op, lds, n, false);
-    store_chunk<double, M>(out, t, v);
-  } else {
-    const XBlock<double, M> xb = x_block<double, M>(geo, t, lds);
-    {
-      double win[M + 2 * H];
-      x_load_window<double, M, H, H>(in, xb, t, win);
-#pragma unroll
-      for (int k = 0; k < M; ++k) v[k] = np_rhs(win, k + H, t.j0 + k, n, r);
+    for (int q = 0; q < M + 2 * H; ++q) {
+      int j = t.j0 - H + q;
+      j = j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
+      win[q] = ld(in, t.at(j));
     }
-    solve_table<double, M, false>(v, t, op, lds, n, false);
-    x_store<double, M>(out, xb, v);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int j = t.j0 + k;
+      double x = 0.0;
+      if constexpr (RHS) {
+        if (j < n) x = np_rhs(win, k + H, j, n, r);
+      } else {
+        if (j < n) x = win[k];
+      }
+      v[k] = x;
+    }
+  }
+  solve_table<double, M, LF>(v, t, op, lds, t.C * M, false);
+  if (t.active) {
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+      if (t.j0 + k < n) st(out, t.at(t.j0 + k), v[k]);
   }
 }
 
@@ -344,9 +357,9 @@ int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double*
   if (c->op[op].bc_unsupported) return X3D_ERR_UNSUPPORTED;
   if (c->op[op].np) {
     // non-periodic first derivatives: der?_ij; Thomas alone: the _12 sweeps
-    if (kind == K_D) return launch_line_np(c, dir, op, in, out);
-    if (kind != K_NONE) return X3D_ERR_UNSUPPORTED;
-    periodic_fix = false;
+    if (kind == K_D) return launch_line_np(c, dir, op, in, out, true);
+    if (kind == K_NONE) return launch_line_np(c, dir, op, in, out, false);
+    return X3D_ERR_UNSUPPORTED;
   }
   if (!c->f64_lines) return X3D_ERR_UNSUPPORTED;
   const LinePlan& P = c->plan[dir];
@@ -433,6 +446,7 @@ NpRows np_rows(const x3d_ctx* c, int dir, int op)
   // first_deriv_exp_, ifirstder = 4 (src/x3d_operator_1d.f90:350-391)
   const double h = dir == 0 ? c->dx : dir == 1 ? c->dy : c->dz;
   NpRows r;
+  r.n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
   r.ncl1 = c->ncl1[dir]; r.ncln = c->ncln[dir]; r.npaire = c->op[op].npaire;
   r.afi = c->sc[dir].afi; r.bfi = c->sc[dir].bfi;
   r.af1 = -(5.0 / 2.0) / h; r.bf1 = 2.0 / h; r.cf1 = 0.5 / h;
@@ -442,30 +456,65 @@ NpRows np_rows(const x3d_ctx* c, int dir, int op)
   return r;
 }
 
-int launch_line_np(x3d_ctx* c, int dir, int op, const double* in, double* out)
+// Chunking of a non-periodic line of any length n >= 8: C chunks (a power of
+// two up to 64) of M <= 16 points with C M >= n and the least padding; y/z
+// take up to 4 chunk groups per wavefront and C / 4 wavefronts per line set.
+int np_plan(int n, bool line_fast, LinePlan* P)
+{
+  if (n < 8) return X3D_ERR_SIZE;
+  int C = 0, M = 0, pad = 1 << 30;
+  for (int c = 1; c <= 64; c <<= 1)
+    for (int m = 1; m <= 16; ++m)
+      if (c * m >= n && c * m - n < pad) { C = c; M = m; pad = c * m - n; }
+  if (!C) return X3D_ERR_UNSUPPORTED;
+  LinePlan p;
+  p.n = n; p.M = M; p.C = C; p.line_fast = line_fast;
+  if (line_fast) {
+    p.G = C < 4 ? C : 4;
+    p.Wc = C / p.G;
+    p.Wl = p.Wc >= 4 ? 1 : 4 / p.Wc;
+  } else {
+    p.G = C;
+    p.Wc = 1;
+    p.Wl = 4;
+  }
+  p.LW = 64 / p.G;
+  *P = p;
+  return X3D_OK;
+}
+
+int launch_line_np(x3d_ctx* c, int dir, int op, const double* in, double* out, bool rhs)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS_ALL || !in || !out) return X3D_ERR_ARG;
   if (!c->f64_lines) return X3D_ERR_UNSUPPORTED;
-  const LinePlan& P = c->plan[dir];
+  const LinePlan& P = c->npplan[dir];
   const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
   if (c->op[op].n != n || P.n != n) return X3D_ERR_SIZE;
-  if (!c->op[op].np || P.M == 0) return X3D_ERR_UNSUPPORTED;
+  if (!c->op[op].np || P.M == 0 || !c->nptab) return X3D_ERR_UNSUPPORTED;
   Geom geo;
   Tiling tl;
   dim3 grid, block;
   size_t smem;
-  launch_dims(c, dir, &geo, &tl, &grid, &block, &smem, sizeof(double));
+  launch_dims_plan(c, dir, P, &geo, &tl, &grid, &block, &smem, sizeof(double));
   tl.xcd = 0;
-  const double* tb = c->tab64 + c->off[op];
-  const OpDev<double> od{tb, tb + n, tb + 2 * n, tb + 3 * n, c->op[op].alfa};
-  const NpRows r = np_rows(c, dir, op);
-#define X3D_NP(MM)                                                                                     \
-  case MM:                                                                                             \
-    if (P.line_fast) hipLaunchKernelGGL((k_der_np<MM, true>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
-    else hipLaunchKernelGGL((k_der_np<MM, false>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
+  if (!P.line_fast) smem = 0;  // x-lines: no staging tile, and the sweeps stay in the wavefront
+  const size_t np = (size_t)P.C * P.M;
+  const double* tb = c->nptab + c->op[op].npoff;
+  const OpDev<double> od{tb, tb + np, tb + 2 * np, nullptr, c->op[op].alfa};
+  NpRows r = np_rows(c, dir, op);
+#define X3D_NP(MM)                                                                                            \
+  case MM:                                                                                                    \
+    if (P.line_fast) {                                                                                        \
+      if (rhs) hipLaunchKernelGGL((k_der_np<MM, true, true>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
+      else hipLaunchKernelGGL((k_der_np<MM, true, false>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
+    } else {                                                                                                  \
+      if (rhs) hipLaunchKernelGGL((k_der_np<MM, false, true>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
+      else hipLaunchKernelGGL((k_der_np<MM, false, false>), grid, block, smem, c->stream, in, out, od, r, geo, tl); \
+    }                                                                                                         \
     break;
   switch (P.M) {
-    X3D_NP(1) X3D_NP(2) X3D_NP(3) X3D_NP(4) X3D_NP(5) X3D_NP(6) X3D_NP(8) X3D_NP(12) X3D_NP(16)
+    X3D_NP(1) X3D_NP(2) X3D_NP(3) X3D_NP(4) X3D_NP(5) X3D_NP(6) X3D_NP(7) X3D_NP(8)
+    X3D_NP(9) X3D_NP(10) X3D_NP(11) X3D_NP(12) X3D_NP(13) X3D_NP(14) X3D_NP(15) X3D_NP(16)
     default: return X3D_ERR_UNSUPPORTED;
   }
 #undef X3D_NP

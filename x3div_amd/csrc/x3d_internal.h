@@ -37,6 +37,7 @@ struct OpHost {
   double alfa = 0.0;
   bool circulant = false;  // tables are the library's periodic [alfa, 1, alfa] ones
   bool np = false;         // non-periodic first derivative (der?_ij): boundary rows, no periodic correction
+  size_t npoff = 0;        // np: its padded f, s, w in x3d_ctx::nptab (each npplan's C*M long)
   bool bc_unsupported = false;  // an operator of a non-periodic direction this library does not build
   std::vector<double> f, s, w, p;
   std::vector<double> f0, s0, w0, p0;  // the library's own tables, kept to recognise them
@@ -75,6 +76,8 @@ struct x3d_ctx {
   hipStream_t stream = nullptr;
   x3d::Scheme sc[3];
   int ncl1[3] = {0, 0, 0}, ncln[3] = {0, 0, 0};  // boundary codes per direction (0 periodic)
+  x3d::LinePlan npplan[3];  // non-periodic directions: lines of any length, padded to C*M
+  double* nptab = nullptr;  // padded Thomas tables of the non-periodic instances
   x3d::OpHost op[X3D_NOPS_ALL];
   // device operator tables: for op o, f,s,w,p at tab64 + off[o] + {0,1,2,3}*n
   double* tab64 = nullptr;
@@ -181,11 +184,14 @@ int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp3
 // (src/x3d_derive.f90:156-217): codes 1 free-slip, 2 Dirichlet at each end,
 // the instance's npaire, and first_deriv_exp_'s constants for the spacing.
 struct NpRows {
-  int ncl1, ncln, npaire;
+  int n, ncl1, ncln, npaire;
   double afi, bfi, af1, bf1, cf1, af2, afm, afn, bfn, cfn;
 };
 NpRows np_rows(const x3d_ctx* c, int dir, int op);
-int launch_line_np(x3d_ctx* c, int dir, int op, const double* in, double* out);
+int np_plan(int n, bool line_fast, LinePlan* P);
+// rhs: der?_ij (boundary rows, then the sweeps); else the _12 sweeps alone on
+// the input (in may equal out)
+int launch_line_np(x3d_ctx* c, int dir, int op, const double* in, double* out, bool rhs);
 int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out,
                 bool periodic_fix);
 // The Thomas sweeps alone (in place) with device tables f, s, w, p at

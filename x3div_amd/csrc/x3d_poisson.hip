@@ -838,6 +838,7 @@ extern "C" int x3d_poisson_tables(const x3d_ctx* c, double* ax, double* bx, doub
                                   double* az, double* bz, double* kre)
 {
   if (!c) return X3D_ERR_ARG;
+  if (!x3d::all_periodic(c)) return X3D_ERR_UNSUPPORTED;  // no Poisson tables were made
   auto cp = [](double* dst, const std::vector<double>& src) {
     if (dst) std::copy(src.begin(), src.end(), dst);
   };
