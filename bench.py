@@ -40,11 +40,12 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256, help="grid of the all-threads CPU-baseline steps")
     ap.add_argument("--cpu-n1", type=int, default=128, help="grid of the one-thread CPU-baseline steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--forms", default="fp32,bfp32,batch,large",
+    ap.add_argument("--forms", default="fp32,bfp32,batch,large,nonperiodic",
                     help="comma list of extra forms to measure beside plain FP64: fp32 (precision "
                          "form at n^3), bfp32 (block-floating-point fields, FP64 arithmetic, at "
                          "n^3), batch (96 uneven grids, FP64 and FP32), large (FP32 at "
-                         "1024x768x768 beside FP64 at 768^3); empty for none")
+                         "1024x768x768 beside FP64 at 768^3), nonperiodic (der?_11/22 at 513^3); "
+                         "empty for none")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch a plain device copy of one field (counter calibration) "
                          "and whole steps, for rocprofv3 --pmc runs")
@@ -422,6 +423,41 @@ def measure_batch(args):
     return out
 
 
+def measure_nonperiodic(args, n: int = 513):
+    """The non-periodic first derivatives (der?_11 and der?_22, both
+    instances) on the reference's n^3 = 513^3 non-periodic grid (lines of
+    2^k + 1 points, padded to 64 chunks of 9): time per application between
+    HIP events, 8 B read + 8 B written per point, against the HBM peak."""
+    import numpy as np
+    import torch
+    from x3div_amd import X3D
+    out = {}
+    for bc in ((1, 1), (2, 2)):
+        ctx = X3D(n, n, n, 2.0, 2.0, 2.0, ncl1=(bc[0],) * 3, ncln=(bc[1],) * 3)
+        u = [ctx.field() for _ in range(3)]
+        ctx.init_fields(*u, kind=2, seed=7, amp=0.0)
+        t = ctx.field()
+        for axis in "xyz":
+            for name in (f"der{axis}", f"der{axis}p"):
+                fn = getattr(ctx, f"der{axis}")
+                fn(t, u[0], name)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(ctx.stream)
+                for _ in range(args.kernel_reps):
+                    fn(t, u[0], name)
+                e1.record(ctx.stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / args.kernel_reps
+                gbs = 16 * n ** 3 / (ms / 1e3) / 1e9
+                out[f"{name}_{bc[0]}{bc[1]}"] = {"ms": round(ms, 4), "GBps": round(gbs, 1),
+                                                  "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        ctx.close()
+        del u, t
+        torch.cuda.empty_cache()
+    return {"grid": [n, n, n], "operators": out}
+
+
 def measure_large(args):
     """FP32 precision form at 1024x768x768 beside plain FP64 at 768^3."""
     import numpy as np
@@ -504,6 +540,9 @@ def main():
         log("batch forms done")
     if "large" in extra:
         forms.update(measure_large(args))
+    if "nonperiodic" in extra:
+        forms["nonperiodic_derivatives"] = measure_nonperiodic(args)
+        log("non-periodic derivatives done")
         log("large grids done")
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_n, args.cpu_n1)
