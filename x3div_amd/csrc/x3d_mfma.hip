@@ -50,7 +50,13 @@ struct CircFrag {
 // slab blockIdx.y (slab stride ss); each wavefront owns RB blocks of 32 rows.
 // x: OUT = IN C^T, IN lines [32 bx, 32 bx + 32) of n contiguous points; each
 // wavefront owns RB blocks of 32 output points.
-template <int RB, bool XDIR>
+// STAGE (n <= 512): the work-group's 32 lines of input are read once, with
+// 16-byte loads along memory order (x: the 32 lines are one contiguous run;
+// y/z: 32-column rows of 128 B), into an LDS block of 32 rows of n + 4
+// floats, keeping the block maximum on the way; the MFMA operands then come
+// from LDS.  Otherwise the input is read twice from memory, once for the
+// maximum and once per MFMA step.
+template <int RB, bool XDIR, bool STAGE>
 __global__ __launch_bounds__(256) void k_circ_mfma(const float* __restrict__ in, float* __restrict__ out,
                                                    const CircFrag* __restrict__ tab, int n, long long rs,
                                                    long long ss, float inv_cscale)
@@ -60,26 +66,51 @@ __global__ __launch_bounds__(256) void k_circ_mfma(const float* __restrict__ in,
   // wavefront at every step: staged once per work-group in LDS
   extern __shared__ unsigned char smem_raw[];
   CircFrag* ltab = reinterpret_cast<CircFrag*>(smem_raw);
+  float* blk = reinterpret_cast<float*>(smem_raw + (size_t)(n >> 4) * 64 * sizeof(CircFrag));
+  const int P = n + 4;  // block row pitch in floats (16-byte multiple)
   for (int q = threadIdx.x; q < (n >> 4) * 64; q += 256) ltab[q] = tab[q];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lc = lane & 31, lk8 = 8 * (lane >> 5), lk = 4 * (lane >> 5);
   long long base;
   if constexpr (XDIR) base = ((long long)blockIdx.x * 32 + lc) * n;  // this lane's line
   else base = (long long)blockIdx.y * ss + (long long)blockIdx.x * 32 + lc;  // this lane's column
-  // the block's largest magnitude: thread (lc, g) scans points g, g + 8, ...
+  // the block's largest magnitude
   float m = 0.f;
-  {
+  if constexpr (STAGE) {
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    if constexpr (XDIR) {
+      const f32x4* src = reinterpret_cast<const f32x4*>(in + (long long)blockIdx.x * 32 * n);
+      for (int q = threadIdx.x; q < 8 * n; q += 256) {
+        const f32x4 x = src[q];
+        const int l = (4 * q) / n, k = 4 * q - l * n;
+        *reinterpret_cast<f32x4*>(blk + l * P + k) = x;
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))));
+      }
+    } else {
+      const float* slab = in + (long long)blockIdx.y * ss + (long long)blockIdx.x * 32;
+      for (int q = threadIdx.x; q < 8 * n; q += 256) {
+        const int k = q >> 3, c4 = 4 * (q & 7);
+        const f32x4 x = *reinterpret_cast<const f32x4*>(slab + (long long)k * rs + c4);
+        blk[(c4 + 0) * P + k] = x.x;
+        blk[(c4 + 1) * P + k] = x.y;
+        blk[(c4 + 2) * P + k] = x.z;
+        blk[(c4 + 3) * P + k] = x.w;
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))));
+      }
+    }
+  } else {
+    // thread (lc, g) scans points g, g + 8, ...
     const int g = threadIdx.x >> 5;
     for (int k = g; k < n; k += 8) {
       const float x = XDIR ? in[base + k] : in[base + (long long)k * rs];
       m = fmaxf(m, fabsf(x));
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
-    if (lane == 0) wmax[wave] = m;
-    __syncthreads();
-    m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
   }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
+  if (lane == 0) wmax[wave] = m;
+  __syncthreads();
+  m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
   int e = 0;
   if (m > 0.f) (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
   const float scale = ldexpf(1.f, 14 - e), unscale = ldexpf(inv_cscale, e - 14);
@@ -90,7 +121,11 @@ __global__ __launch_bounds__(256) void k_circ_mfma(const float* __restrict__ in,
     for (int v = 0; v < 16; ++v) acc[r][v] = 0.f;
   for (int k0 = 0; k0 < n; k0 += 16) {
     float x[8];
-    if constexpr (XDIR) {
+    if constexpr (STAGE) {
+      const float* row = blk + lc * P + k0 + lk8;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) x[v] = row[v] * scale;
+    } else if constexpr (XDIR) {
 #pragma unroll
       for (int v = 0; v < 8; ++v) x[v] = in[base + k0 + lk8 + v] * scale;
     } else {
@@ -233,19 +268,33 @@ extern "C" int x3d_apply_circ_mfma(x3d_ctx* c, int op, float* out, const float* 
   }
   const int RB = n / 128;
   const float inv = c->circ_scale[op];  // undoes the host scaling of C
-  const size_t smem = (size_t)(n / 16) * 64 * sizeof(CircFrag);  // 128 n bytes
+  const size_t tsmem = (size_t)(n / 16) * 64 * sizeof(CircFrag);  // 128 n bytes
+  // lines of up to 512 points: the input block too (32 rows of n + 4 floats)
+  const bool stage = n <= 512;
+  const size_t smem = tsmem + (stage ? (size_t)32 * (n + 4) * sizeof(float) : 0);
 #define X3D_RB(R)                                                                                         \
   case R:                                                                                                 \
-    if (smem > 65536) {                                                                                   \
-      (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, true>,                                        \
+    if (stage) {                                                                                          \
+      (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, true, true>,                                  \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);                   \
-      (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, false>,                                       \
+      (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, false, true>,                                 \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);                   \
+      if (dir == 0)                                                                                       \
+        hipLaunchKernelGGL((k_circ_mfma<R, true, true>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
+      else                                                                                                \
+        hipLaunchKernelGGL((k_circ_mfma<R, false, true>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
+    } else {                                                                                              \
+      if (smem > 65536) {                                                                                 \
+        (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, true, false>,                               \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);                 \
+        (void)hipFuncSetAttribute((const void*)k_circ_mfma<R, false, false>,                              \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);                 \
+      }                                                                                                   \
+      if (dir == 0)                                                                                       \
+        hipLaunchKernelGGL((k_circ_mfma<R, true, false>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
+      else                                                                                                \
+        hipLaunchKernelGGL((k_circ_mfma<R, false, false>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
     }                                                                                                     \
-    if (dir == 0)                                                                                         \
-      hipLaunchKernelGGL((k_circ_mfma<R, true>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
-    else                                                                                                  \
-      hipLaunchKernelGGL((k_circ_mfma<R, false>), grid, dim3(256), smem, c->stream, in, out, tab, n, rs, ss, inv); \
     break;
   switch (RB) {
     X3D_RB(1) X3D_RB(2) X3D_RB(3) X3D_RB(4) X3D_RB(5) X3D_RB(6) X3D_RB(7) X3D_RB(8)
