@@ -1,7 +1,8 @@
 """Poisson with the fused z stage (2-D hipFFT over x, y and one kernel for the
 forward z DFT, the division and the inverse z DFT) against the oracle, for
-every power-of-two z length the kernel is built for (radix-8 passes with a
-final radix-4 or radix-2 pass), in both precisions.  FP64 at the 1e-12 bar,
+every z length the kernel is built for (radix-8 passes with a final radix-4
+or radix-2 pass, then radix-3 and radix-5 passes for 3- and 5-smooth
+lengths), in both precisions.  FP64 at the 1e-12 bar,
 FP32 at 2e-6 (the precision form's Poisson bound).
 
 Lines of up to 512 points run the persistent, pipelined z kernel (each
@@ -18,7 +19,9 @@ from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-NZ = [8, 16, 32, 64, 128, 256, 512, 1024]
+# powers of two (radix-8 passes, then radix-4 or radix-2) and 3- and 5-smooth
+# lengths (radix-3 and radix-5 passes after those)
+NZ = [8, 16, 32, 64, 128, 256, 512, 1024, 24, 40, 48, 96, 160, 384, 768]
 
 
 @pytest.mark.parametrize("nz", NZ)
@@ -47,7 +50,7 @@ def test_poisson_fused_z(torch_cuda, nz):
     assert e32 < 2e-6, e32
 
 
-@pytest.mark.parametrize("ny", [8, 32, 128, 512, 1024])
+@pytest.mark.parametrize("ny", [8, 32, 128, 512, 1024, 24, 48, 80, 96, 160, 192, 320])
 def test_poisson_fused_z_y_lengths(torch_cuda, ny):
     """The fused z stage under 2-D plans of several y lengths (the 2-D
     hipFFT layout of the half spectrum that the z kernel reads)."""

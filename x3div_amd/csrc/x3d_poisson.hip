@@ -93,12 +93,50 @@ __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - 
 template <typename C>
 __device__ __forceinline__ C cmi(C a, bool inv) { C r; if (inv) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; } return r; }
 
-// In-place DFT of length R (2, 4, 8) with sign -1 (forward) or +1 (inverse).
+// In-place DFT of length R (2, 3, 4, 5, 8) with sign -1 (forward) or +1
+// (inverse).
 template <typename C, int R>
 __device__ __forceinline__ void small_dft(C (&v)[R], bool inv)
 {
   using T = typename CplxT<C>::R;
-  if constexpr (R == 2) {
+  if constexpr (R == 3) {
+    // X1,2 = a - (b + c)/2 -/+ i (sqrt 3 / 2)(b - c) (forward)
+    const T s3 = (T)0.86602540378443864676;
+    const C a = v[0], t1 = cadd(v[1], v[2]);
+    C m;
+    m.x = a.x - (T)0.5 * t1.x;
+    m.y = a.y - (T)0.5 * t1.y;
+    C n = cmi(csub(v[1], v[2]), inv);
+    n.x = s3 * n.x;
+    n.y = s3 * n.y;
+    v[0] = cadd(a, t1);
+    v[1] = cadd(m, n);
+    v[2] = csub(m, n);
+  } else if constexpr (R == 5) {
+    // t1 = b + e, t2 = c + d, t3 = b - e, t4 = c - d;
+    // X1,4 = a + c1 t1 + c2 t2 -/+ i (s1 t3 + s2 t4),
+    // X2,3 = a + c2 t1 + c1 t2 -/+ i (s2 t3 - s1 t4)   (forward)
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+    const C a = v[0];
+    const C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    C m1, m2, n1, n2;
+    m1.x = a.x + c1 * t1.x + c2 * t2.x;
+    m1.y = a.y + c1 * t1.y + c2 * t2.y;
+    m2.x = a.x + c2 * t1.x + c1 * t2.x;
+    m2.y = a.y + c2 * t1.y + c1 * t2.y;
+    n1.x = s1 * t3.x + s2 * t4.x;
+    n1.y = s1 * t3.y + s2 * t4.y;
+    n2.x = s2 * t3.x - s1 * t4.x;
+    n2.y = s2 * t3.y - s1 * t4.y;
+    n1 = cmi(n1, inv);
+    n2 = cmi(n2, inv);
+    v[0] = cadd(a, cadd(t1, t2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+  } else if constexpr (R == 2) {
     const C a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -192,18 +230,42 @@ __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict_
   __syncthreads();
 }
 
+// Lengths N = 2^a 3^b 5^c: the power-of-two part in radix-8 passes (then one
+// radix-4 or radix-2 pass), then radix-3 and radix-5 passes; Stockham's
+// autosort works for any order of the factors.
+constexpr int pow_part(int n, int p) { return n % p ? 1 : p * pow_part(n / p, p); }
+constexpr int count_factor(int n, int p) { return n % p ? 0 : 1 + count_factor(n / p, p); }
+constexpr bool smooth235(int n) { return n == pow_part(n, 2) * pow_part(n, 3) * pow_part(n, 5); }
+
 template <typename C, int N, int TI, int NT>
 __device__ __forceinline__ void line_dft(C* buf, const double2* __restrict__ tw, bool inv)
 {
+  static_assert(smooth235(N), "line lengths are 2^a 3^b 5^c");
   int Ns = 1;
-  constexpr int L = __builtin_ctz(N);
+  constexpr int L = __builtin_ctz(N), L3 = count_factor(N, 3), L5 = count_factor(N, 5);
 #pragma unroll
   for (int p = 0; p < L / 3; ++p) {
     stockham_pass<C, N, 8, TI, NT>(buf, tw, Ns, inv);
     Ns *= 8;
   }
-  if constexpr (L % 3 == 2) stockham_pass<C, N, 4, TI, NT>(buf, tw, Ns, inv);
-  if constexpr (L % 3 == 1) stockham_pass<C, N, 2, TI, NT>(buf, tw, Ns, inv);
+  if constexpr (L % 3 == 2) {
+    stockham_pass<C, N, 4, TI, NT>(buf, tw, Ns, inv);
+    Ns *= 4;
+  }
+  if constexpr (L % 3 == 1) {
+    stockham_pass<C, N, 2, TI, NT>(buf, tw, Ns, inv);
+    Ns *= 2;
+  }
+#pragma unroll
+  for (int p = 0; p < L3; ++p) {
+    stockham_pass<C, N, 3, TI, NT>(buf, tw, Ns, inv);
+    Ns *= 3;
+  }
+#pragma unroll
+  for (int p = 0; p < L5; ++p) {
+    stockham_pass<C, N, 5, TI, NT>(buf, tw, Ns, inv);
+    Ns *= 5;
+  }
 }
 
 // TI neighbouring x-columns per work-group of NT threads
@@ -527,6 +589,8 @@ static int spl_prepare(x3d_ctx* c, int n)
 #define X3D_SPL_N(NN) \
     case NN: if constexpr (spl_ok<C, NN, MODE>()) return spl_prepare_n<C, NN, MODE>(c); break;
     X3D_SPL_N(8) X3D_SPL_N(16) X3D_SPL_N(32) X3D_SPL_N(64) X3D_SPL_N(128) X3D_SPL_N(256) X3D_SPL_N(512)
+    X3D_SPL_N(24) X3D_SPL_N(40) X3D_SPL_N(48) X3D_SPL_N(80) X3D_SPL_N(96) X3D_SPL_N(160) X3D_SPL_N(192)
+    X3D_SPL_N(320) X3D_SPL_N(384)
 #undef X3D_SPL_N
     default: break;
   }
@@ -587,6 +651,15 @@ static int launch_poisson_y(x3d_ctx* c, C* spec)
     case 128: return launch_spl<C, 128, 2>(c, spec, c->ytw);
     case 256: return launch_spl<C, 256, 2>(c, spec, c->ytw);
     case 512: return launch_spl<C, 512, 2>(c, spec, c->ytw);
+    case 24: return launch_spl<C, 24, 2>(c, spec, c->ytw);
+    case 40: return launch_spl<C, 40, 2>(c, spec, c->ytw);
+    case 48: return launch_spl<C, 48, 2>(c, spec, c->ytw);
+    case 80: return launch_spl<C, 80, 2>(c, spec, c->ytw);
+    case 96: return launch_spl<C, 96, 2>(c, spec, c->ytw);
+    case 160: return launch_spl<C, 160, 2>(c, spec, c->ytw);
+    case 192: return launch_spl<C, 192, 2>(c, spec, c->ytw);
+    case 320: return launch_spl<C, 320, 2>(c, spec, c->ytw);
+    case 384: return launch_spl<C, 384, 2>(c, spec, c->ytw);
     default: return X3D_ERR_UNSUPPORTED;
   }
 }
@@ -603,14 +676,35 @@ static int launch_poisson_z(x3d_ctx* c, C* spec)
     case 256: return launch_pz_n<C, 256>(c, spec);
     case 512: return launch_pz_n<C, 512>(c, spec);
     case 1024: return launch_pz_n<C, 1024>(c, spec);
+    case 24: return launch_pz_n<C, 24>(c, spec);
+    case 40: return launch_pz_n<C, 40>(c, spec);
+    case 48: return launch_pz_n<C, 48>(c, spec);
+    case 80: return launch_pz_n<C, 80>(c, spec);
+    case 96: return launch_pz_n<C, 96>(c, spec);
+    case 160: return launch_pz_n<C, 160>(c, spec);
+    case 192: return launch_pz_n<C, 192>(c, spec);
+    case 320: return launch_pz_n<C, 320>(c, spec);
+    case 384: return launch_pz_n<C, 384>(c, spec);
+    case 640: return launch_pz_n<C, 640>(c, spec);
+    case 768: return launch_pz_n<C, 768>(c, spec);
     default: return X3D_ERR_UNSUPPORTED;
   }
 }
 
 #define X3D_TRY_RC(expr) do { const int _r = (expr); if (_r) return _r; } while (0)
 
-static bool zsplit_ok(int nz) { return nz >= 8 && nz <= 1024 && (nz & (nz - 1)) == 0; }
-static bool ysplit_ok(int ny) { return ny >= 8 && ny <= 512 && (ny & (ny - 1)) == 0; }
+// the lengths the line kernels are built for: powers of two and these
+// 3- and 5-smooth ones
+static bool smooth_len(int n)
+{
+  static const int extra[] = {24, 40, 48, 80, 96, 160, 192, 320, 384, 640, 768};
+  if ((n & (n - 1)) == 0) return true;
+  for (int e : extra)
+    if (e == n) return true;
+  return false;
+}
+static bool zsplit_ok(int nz) { return nz >= 8 && nz <= 1024 && smooth_len(nz); }
+static bool ysplit_ok(int ny) { return ny >= 8 && ny <= 512 && smooth_len(ny); }
 
 // Table of e^{-2 pi i t / n}, t < n, on the device.
 static int twiddles(int n, void** dev)
