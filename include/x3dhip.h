@@ -89,6 +89,23 @@ int x3d_create(int nx, int ny, int nz, double xlx, double yly, double zlz, x3d_c
  * src/poisson.f90:104-108). */
 int x3d_create_bc(int nx, int ny, int nz, double xlx, double yly, double zlz,
                   const int *ncl1, const int *ncln, x3d_ctx **ctx);
+/* Grid stretching in y: istret and the vectors ppy, ppyi of module param
+ * (src/module_param.f90:70-71,104; allocated at src/var.f90:138-140).  With
+ * istret /= 0 the reference multiplies the results of dery by ppy(j)
+ * (src/x3d_derive.f90:331-337, 427-433), of deryvp by ppyi(j) and of derypv
+ * by ppy(j) (src/x3d_staggered.f90:628-634, 874-880).  ppy and ppyi are host
+ * vectors of ny values each, copied to the device here; istret = 0 turns the
+ * scaling off (the vectors may then be NULL).  x3d_dery, x3d_deryvp,
+ * x3d_derypv and x3d_dery_f32 scale by the vector they are given (one that
+ * differs from the context's replaces it; NULL keeps the context's);
+ * x3d_apply_f32, x3d_apply_bfp, the steps and their pieces scale by the
+ * context's.  Interpolations and Poisson are unchanged (the reference's
+ * kxyz uses yk2, which istret leaves alone, src/poisson.f90:562-565).  On a
+ * stretched context the second derivative in y (a stretched grid needs
+ * pp2y/pp4y), the matrix-core y-derivatives and batches return
+ * X3D_ERR_UNSUPPORTED.  A captured step is released.  The reference's x3div
+ * driver fixes istret = 0 (src/parameters.f90:117). */
+int x3d_set_stretching(x3d_ctx *ctx, int istret, const double *ppy, const double *ppyi);
 /* x3d_operator_1d_finalize + decomp_2d_poisson_finalize (src/poisson.f90:237-271) */
 int x3d_destroy(x3d_ctx *ctx);
 int x3d_set_stream(x3d_ctx *ctx, void *hip_stream);
@@ -148,8 +165,8 @@ int x3d_thomas1d(double *tt, const double *ff, const double *fs, const double *f
  *      abstract interfaces src/x3d_derive.f90:18-51; periodic derx_00,
  *      dery_00, derz_00 at :111-152, :279-339, :503-558) ------------------ */
 int x3d_derx(x3d_ctx *ctx, double *tx, const double *ux, int op, int nx, int ny, int nz);
-/* ppy is the stretching vector of DERIVATIVE_Y; istret = 0 (src/parameters.f90:117)
- * so it is accepted and unused, as in the reference. May be NULL. */
+/* ppy: the stretching vector of DERIVATIVE_Y (host, ny values), used when
+ * the context is stretched (x3d_set_stretching); may be NULL. */
 int x3d_dery(x3d_ctx *ctx, double *ty, const double *uy, int op, const double *ppy,
              int nx, int ny, int nz);
 int x3d_derz(x3d_ctx *ctx, double *tz, const double *uz, int op, int nx, int ny, int nz);

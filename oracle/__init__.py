@@ -88,6 +88,8 @@ def lib(fast: bool = False):
         L.orc_np_tables.restype = C.c_int
         L.orc_der_np.argtypes = [dp, dp] + [C.c_int] * 4 + [C.c_double] + [C.c_int] * 3 + [dp] * 3
         L.orc_der_np.restype = C.c_int
+        L.orc_set_stretching.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_set_stretching.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
         _libs[fast] = L
@@ -192,6 +194,17 @@ class Oracle:
         sizes = [self.nx, self.nx, self.ny, self.ny, self.nz, self.nz]
         parts = np.split(out, np.cumsum(sizes)[:-1])
         return dict(zip(["ax", "bx", "ay", "by", "az", "bz"], parts))
+
+    def set_stretching(self, istret: int, ppy=None, ppyi=None) -> None:
+        """istret != 0: y-derivatives scaled by ppy(j) (dery, derypv) or ppyi(j) (deryvp)."""
+        if istret:
+            a = np.ascontiguousarray(ppy, dtype=np.float64)
+            b = np.ascontiguousarray(ppyi, dtype=np.float64)
+            assert a.size == self.ny and b.size == self.ny
+            rc = self._L.orc_set_stretching(self._c, int(istret), _p(a), _p(b))
+        else:
+            rc = self._L.orc_set_stretching(self._c, 0, None, None)
+        assert rc == 0
 
     def apply(self, name: str, u: np.ndarray, kind: str | None = None) -> np.ndarray:
         """Apply one operator (der?/der?vp/der?pv/inter?vp/inter?pv) along its axis."""

@@ -259,6 +259,7 @@ void orc_free(orc_ctx *c)
   free(c->ax); free(c->bx); free(c->ay); free(c->by); free(c->az); free(c->bz);
   free(c->xk2); free(c->exs); free(c->yk2); free(c->eys); free(c->zk2); free(c->ezs);
   free(c->kxyz);
+  free(c->ppy); free(c->ppyi);
   memset(c, 0, sizeof(*c));
 }
 
@@ -482,6 +483,33 @@ static void rhs_run(double *t, const double *const *U, int len, int kind,
   }
 }
 
+int orc_set_stretching(orc_ctx *c, int istret, const double *ppy, const double *ppyi)
+{
+  if (istret && (!ppy || !ppyi)) return -1;
+  free(c->ppy); free(c->ppyi);
+  c->ppy = c->ppyi = NULL;
+  c->istret = istret;
+  if (!istret) return 0;
+  c->ppy = malloc(sizeof(double) * (size_t)c->ny);
+  c->ppyi = malloc(sizeof(double) * (size_t)c->ny);
+  memcpy(c->ppy, ppy, sizeof(double) * (size_t)c->ny);
+  memcpy(c->ppyi, ppyi, sizeof(double) * (size_t)c->ny);
+  return 0;
+}
+
+/* The stretching loop after the y solve: dery_00 / dery_ij
+ * (src/x3d_derive.f90:331-337, 427-433: ppy), deryvp (src/x3d_staggered.f90:628-634:
+ * ppyi), derypv (:874-880: ppy).  Interpolations are not scaled. */
+static void stretch_y(const orc_ctx *c, double *t, int kind, int nx, int ny, int nz)
+{
+  if (!c->istret || !(kind == 0 || kind == 1 || kind == 2)) return;
+  const double *s = kind == 1 ? c->ppyi : c->ppy;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) t[IDX(i, j, k, nx, ny)] = t[IDX(i, j, k, nx, ny)] * s[j];
+}
+
 int orc_apply(const orc_ctx *c, double *t, const double *u, int op, int dir,
               int kind, int nx, int ny, int nz)
 {
@@ -514,11 +542,13 @@ int orc_apply(const orc_ctx *c, double *t, const double *u, int op, int dir,
         rhs_run(t + q, U, nx, kind, a, b, cc, d);
       }
     }
-  return orc_thomas(c, t, op, dir, nx, ny, nz);
+  const int rc = orc_thomas(c, t, op, dir, nx, ny, nz);
+  if (rc == 0 && dir == 1) stretch_y(c, t, kind, nx, ny, nz);
+  return rc;
 }
 
 /* derx_00 / dery_00 / derz_00: src/x3d_derive.f90:111-152, 279-339, 503-558
- * (istret = 0, so dery's ppy scaling is not applied). */
+ * (dery's ppy scaling when the context is stretched, in orc_apply). */
 void orc_derx(const orc_ctx *c, double *t, const double *u, int op, int nx, int ny, int nz)
 { orc_apply(c, t, u, op, 0, 0, nx, ny, nz); }
 void orc_dery(const orc_ctx *c, double *t, const double *u, int op, int nx, int ny, int nz)

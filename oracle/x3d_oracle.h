@@ -54,6 +54,9 @@ typedef struct {
   double *xk2, *yk2, *zk2, *exs, *eys, *ezs;  /* real parts (imag parts equal) */
   double *kxyz;   /* complex nx*ny*(nz/2+1), interleaved */
   double eps;     /* 1e-16, src/poisson.f90:24-28 */
+  /* y-stretching: istret and ppy, ppyi (ny each), src/module_param.f90:70-71,104 */
+  int istret;
+  double *ppy, *ppyi;
 } orc_ctx;
 
 /* ---- setup --------------------------------------------------------------- */
@@ -89,6 +92,9 @@ int  orc_thomas(const orc_ctx *c, double *tt, int op, int dir, int nx, int ny, i
 /* kind: 0 = D (der?_00), 1 = Ds v->p, 2 = Ds p->v, 3 = I v->p, 4 = I p->v     */
 int  orc_apply(const orc_ctx *c, double *t, const double *u, int op, int dir,
                int kind, int nx, int ny, int nz);
+/* istret /= 0: y-derivatives are multiplied by ppy(j) (dery, derypv) or
+ * ppyi(j) (deryvp) after the solve; vectors of ny values, copied. */
+int  orc_set_stretching(orc_ctx *c, int istret, const double *ppy, const double *ppyi);
 void orc_derx(const orc_ctx *c, double *t, const double *u, int op, int nx, int ny, int nz);
 void orc_dery(const orc_ctx *c, double *t, const double *u, int op, int nx, int ny, int nz);
 void orc_derz(const orc_ctx *c, double *t, const double *u, int op, int nx, int ny, int nz);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "x3d_step_f32": (_i, [_vp, _d] + [_dp] * 10),
     "x3d_step_timed_f32": (_i, [_vp, _d] + [_dp] * 10 + [C.POINTER(C.c_float)]),
     "x3d_create_bc": (_i, [_i, _i, _i, _d, _d, _d, _ip, _ip, C.POINTER(_vp)]),
+    "x3d_set_stretching": (_i, [_vp, _i, _vp, _vp]),
     "x3d_bfp_encode": (_i, [_vp, _dp, _dp]),
     "x3d_bfp_decode": (_i, [_vp, _dp, _dp]),
     "x3d_apply_bfp": (_i, [_vp, _i, _dp, _dp]),

@@ -50,6 +50,15 @@ def _ptr(t, shape=None, dtype=None) -> int:
     return t.data_ptr()
 
 
+def _host_vec(v, n: int, dtype=np.float64) -> np.ndarray:
+    """A contiguous host copy of a length-n coefficient vector (the caller's
+    array must not be freed while the library reads it)."""
+    a = np.ascontiguousarray(np.asarray(v, dtype=dtype).reshape(-1))
+    if a.size != n:
+        raise ValueError(f"vector of length {a.size}, expected {n}")
+    return a
+
+
 class X3D:
     """A periodic nx*ny*nz grid of lengths (xlx, yly, zlz) on the GPU."""
 
@@ -175,9 +184,22 @@ class X3D:
     def derx(self, tx, ux, op="derx"):
         check("x3d_derx", self.lib.x3d_derx(self._h, self._f(tx), self._f(ux), _op_id(op), self.nx, self.ny, self.nz))
 
+    # ---- y-stretching (istret, ppy, ppyi: src/module_param.f90:70-71,104) ----------
+    def set_stretching(self, istret: int, ppy=None, ppyi=None):
+        """istret != 0: y-derivatives (dery, deryvp, derypv and their uses in
+        the step) are multiplied by ppy(j), deryvp by ppyi(j); host vectors of
+        ny values.  istret = 0 turns it off."""
+        if istret:
+            self._sy = [_host_vec(ppy, self.ny), _host_vec(ppyi, self.ny)]
+            a, b = (v.ctypes.data for v in self._sy)
+        else:
+            a = b = None
+        check("x3d_set_stretching", self.lib.x3d_set_stretching(self._h, int(istret), a, b))
+
     def dery(self, ty, uy, op="dery", ppy=None):
-        check("x3d_dery", self.lib.x3d_dery(self._h, self._f(ty), self._f(uy), _op_id(op), None,
-                                            self.nx, self.ny, self.nz))
+        v = None if ppy is None else _host_vec(ppy, self.ny)
+        check("x3d_dery", self.lib.x3d_dery(self._h, self._f(ty), self._f(uy), _op_id(op),
+                                            None if v is None else v.ctypes.data, self.nx, self.ny, self.nz))
 
     def derz(self, tz, uz, op="derz"):
         check("x3d_derz", self.lib.x3d_derz(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz))
@@ -209,13 +231,17 @@ class X3D:
         check("x3d_interyvp", self.lib.x3d_interyvp(self._h, self._f(ty), self._f(uy), _op_id(op), self.nx, self.ny, self.ny, self.nz))
 
     def deryvp(self, ty, uy, op="deryvp", ppyi=None):
-        check("x3d_deryvp", self.lib.x3d_deryvp(self._h, self._f(ty), self._f(uy), _op_id(op), None, self.nx, self.ny, self.ny, self.nz))
+        v = None if ppyi is None else _host_vec(ppyi, self.ny)
+        check("x3d_deryvp", self.lib.x3d_deryvp(self._h, self._f(ty), self._f(uy), _op_id(op),
+                                                None if v is None else v.ctypes.data, self.nx, self.ny, self.ny, self.nz))
 
     def interypv(self, ty, uy, op="intypv"):
         check("x3d_interypv", self.lib.x3d_interypv(self._h, self._f(ty), self._f(uy), _op_id(op), self.nx, self.ny, self.ny, self.nz))
 
     def derypv(self, ty, uy, op="derypv", ppy=None):
-        check("x3d_derypv", self.lib.x3d_derypv(self._h, self._f(ty), self._f(uy), _op_id(op), None, self.nx, self.ny, self.ny, self.nz))
+        v = None if ppy is None else _host_vec(ppy, self.ny)
+        check("x3d_derypv", self.lib.x3d_derypv(self._h, self._f(ty), self._f(uy), _op_id(op),
+                                                None if v is None else v.ctypes.data, self.nx, self.ny, self.ny, self.nz))
 
     def derzvp(self, tz, uz, op="derzvp"):
         check("x3d_derzvp", self.lib.x3d_derzvp(self._h, self._f(tz), self._f(uz), _op_id(op), self.nx, self.ny, self.nz, self.nz))
@@ -295,9 +321,10 @@ class X3D:
         check("x3d_derx_f32", self.lib.x3d_derx_f32(self._h, self._f32(tx), self._f32(ux), _op_id(op),
                                                     self.nx, self.ny, self.nz))
 
-    def dery_f32(self, ty, uy, op="dery"):
-        check("x3d_dery_f32", self.lib.x3d_dery_f32(self._h, self._f32(ty), self._f32(uy), _op_id(op), None,
-                                                    self.nx, self.ny, self.nz))
+    def dery_f32(self, ty, uy, op="dery", ppy=None):
+        v = None if ppy is None else _host_vec(ppy, self.ny, np.float32)
+        check("x3d_dery_f32", self.lib.x3d_dery_f32(self._h, self._f32(ty), self._f32(uy), _op_id(op),
+                                                    None if v is None else v.ctypes.data, self.nx, self.ny, self.nz))
 
     def derz_f32(self, tz, uz, op="derz"):
         check("x3d_derz_f32", self.lib.x3d_derz_f32(self._h, self._f32(tz), self._f32(uz), _op_id(op),

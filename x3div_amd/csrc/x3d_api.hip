@@ -588,6 +588,8 @@ int x3d_destroy(x3d_ctx* c)
     if (t) (void)hipFree(t);
   if (c->usertab) (void)hipFree(c->usertab);
   if (c->nptab) (void)hipFree(c->nptab);
+  if (c->sy64) (void)hipFree(c->sy64);
+  if (c->sy32) (void)hipFree(c->sy32);
   if (c->cap_stream) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(c->cap_stream, &g);
@@ -711,6 +713,53 @@ static int check3(const x3d_ctx* c, int nx, int ny, int nz)
   return (nx == c->nx && ny == c->ny && nz == c->nz) ? X3D_OK : X3D_ERR_SIZE;
 }
 
+// ---- y-stretching (istret, ppy, ppyi: src/parameters.f90:117, src/var.f90:138-140)
+// Host vectors, ny values each, kept on the device in FP64 and FP32.  The
+// upload waits for the stream first: earlier work may still read the vectors.
+static int stretch_upload(x3d_ctx* c, const double* ppy, const double* ppyi)
+{
+  const size_t ny = (size_t)c->ny;
+  std::vector<double> h(2 * ny);
+  std::memcpy(h.data(), ppy, ny * sizeof(double));
+  std::memcpy(h.data() + ny, ppyi, ny * sizeof(double));
+  std::vector<float> h32(h.begin(), h.end());
+  if (!c->sy64) X3D_CHECK_HIP(hipMalloc(&c->sy64, 2 * ny * sizeof(double)));
+  if (!c->sy32) X3D_CHECK_HIP(hipMalloc(&c->sy32, 2 * ny * sizeof(float)));
+  X3D_CHECK_HIP(hipStreamSynchronize(c->stream));
+  X3D_CHECK_HIP(hipMemcpy(c->sy64, h.data(), 2 * ny * sizeof(double), hipMemcpyHostToDevice));
+  X3D_CHECK_HIP(hipMemcpy(c->sy32, h32.data(), 2 * ny * sizeof(float), hipMemcpyHostToDevice));
+  c->sy_host = std::move(h);
+  return X3D_OK;
+}
+
+// The vector a y-derivative entry point was given (which = 0 ppy, 1 ppyi):
+// with istret /= 0 the reference multiplies by the vector passed, so one
+// that differs from the context's replaces it (both vectors stay in use by
+// the step passes).  NULL keeps the context's; istret = 0 ignores it.
+static int stretch_arg(x3d_ctx* c, int which, const double* v)
+{
+  if (!c->istret || !v) return X3D_OK;
+  const size_t ny = (size_t)c->ny;
+  const double* cur = c->sy_host.data() + which * ny;
+  if (std::memcmp(cur, v, ny * sizeof(double)) == 0) return X3D_OK;
+  const std::vector<double> keep(c->sy_host);
+  graph_release(c);
+  return which == 0 ? stretch_upload(c, v, keep.data() + ny) : stretch_upload(c, keep.data(), v);
+}
+
+int x3d_set_stretching(x3d_ctx* c, int istret, const double* ppy, const double* ppyi)
+{
+  if (!c || (istret && (!ppy || !ppyi))) return X3D_ERR_ARG;
+  // a captured step holds the previous state
+  graph_release(c);
+  if (istret) {
+    const int rc = stretch_upload(c, ppy, ppyi);
+    if (rc) return rc;
+  }
+  c->istret = istret;
+  return X3D_OK;
+}
+
 int x3d_xthomas(x3d_ctx* c, double* tt, int op, int nx, int ny, int nz)
 {
   int rc = check3(c, nx, ny, nz);
@@ -799,8 +848,8 @@ int x3d_derx(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int
 }
 int x3d_dery(x3d_ctx* c, double* t, const double* u, int op, const double* ppy, int nx, int ny, int nz)
 {
-  (void)ppy;
   int rc = check3(c, nx, ny, nz);
+  if (!rc) rc = stretch_arg(c, 0, ppy);
   return rc ? rc : launch_line(c, 1, K_D, op, u, t, true);
 }
 int x3d_derz(x3d_ctx* c, double* t, const double* u, int op, int nx, int ny, int nz)
@@ -851,19 +900,19 @@ X3D_STAG(x3d_interzpv, 2, K_IPV, nx, ny, nzm, nz, nx == c->nx && ny == c->ny && 
 int x3d_deryvp(x3d_ctx* c, double* t, const double* u, int op, const double* ppyi, int nx, int ny,
                int nym, int nz)
 {
-  (void)ppyi;
   if (!c) return X3D_ERR_ARG;
   if (!(nx == c->nx && ny == c->ny && nym == ny && nz == c->nz)) return X3D_ERR_SIZE;
-  return launch_line(c, 1, K_DVP, op, u, t, true);
+  const int rc = stretch_arg(c, 1, ppyi);
+  return rc ? rc : launch_line(c, 1, K_DVP, op, u, t, true);
 }
 
 int x3d_derypv(x3d_ctx* c, double* t, const double* u, int op, const double* ppy, int nx, int nym,
                int ny, int nz)
 {
-  (void)ppy;
   if (!c) return X3D_ERR_ARG;
   if (!(nx == c->nx && ny == c->ny && nym == ny && nz == c->nz)) return X3D_ERR_SIZE;
-  return launch_line(c, 1, K_DPV, op, u, t, true);
+  const int rc = stretch_arg(c, 0, ppy);
+  return rc ? rc : launch_line(c, 1, K_DPV, op, u, t, true);
 }
 
 // ---- transposes ------------------------------------------------------------
@@ -1073,8 +1122,11 @@ int x3d_derx_f32(x3d_ctx* c, float* t, const float* u, int op, int nx, int ny, i
 }
 int x3d_dery_f32(x3d_ctx* c, float* t, const float* u, int op, const float* ppy, int nx, int ny, int nz)
 {
-  (void)ppy;
   int rc = check3(c, nx, ny, nz);
+  if (!rc && c->istret && ppy) {
+    const std::vector<double> v(ppy, ppy + c->ny);
+    rc = stretch_arg(c, 0, v.data());
+  }
   return rc ? rc : launch_line_f32(c, 1, K_D, op, u, t);
 }
 int x3d_derz_f32(x3d_ctx* c, float* t, const float* u, int op, int nx, int ny, int nz)

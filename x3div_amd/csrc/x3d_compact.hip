@@ -351,7 +351,7 @@ static hipError_t launch_any(int kind, int M, dim3 grid, dim3 block, size_t smem
   }
 }
 
-int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out, bool periodic_fix)
+static int line_raw(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out, bool periodic_fix)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS_ALL || !in || !out) return X3D_ERR_ARG;
   if (c->op[op].bc_unsupported) return X3D_ERR_UNSUPPORTED;
@@ -418,7 +418,7 @@ int launch_thomas_tables(x3d_ctx* c, int dir, const double* tab, double alfa, do
 
 // FP32 precision form of one operator: the circulant solver in FP32 (the
 // library's periodic tables only; no Thomas-table path).
-int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out)
+static int line_raw_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS_ALL || !in || !out || kind == K_NONE) return X3D_ERR_ARG;
   // y/z: the FP32 plans' tiling (chunks up to 32 points, 128-byte rows of 32
@@ -524,7 +524,7 @@ int launch_line_np(x3d_ctx* c, int dir, int op, const double* in, double* out, b
 
 // BFP32 form of one operator: BFP32 fields, the circulant solver in FP64 on
 // the FP64 plans (y/z-line lane groups of 16 lines: one segment per row).
-int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp32* out)
+static int line_raw_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp32* out)
 {
   if (dir < 0 || dir > 2 || op < 0 || op >= X3D_NOPS_ALL || !in || !out || kind == K_NONE) return X3D_ERR_ARG;
   if (c->nx % 16) return X3D_ERR_UNSUPPORTED;
@@ -547,5 +547,88 @@ int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp3
   const hipError_t e = launch_any<bfp32, true>(kind, P.M, grid, block, smem, c->stream, in, out, od, cf, cc, geo, tl);
   return e == hipSuccess ? X3D_OK : hip_status(e);
 }
+
+// ---- y-stretching: the reference's separate loop after the line solve --------
+// out(i, j, k) *= s(j) over the whole field; 256-thread blocks, a grid-stride
+// loop.  BFP32: each 16-point x-segment lies in one y-row, so it is decoded,
+// scaled by its row's factor and encoded again with a new exponent.
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale_y(T* __restrict__ out, const T* __restrict__ s, int nx, int ny,
+                                                 long long n)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = out[i] * s[(i / nx) % ny];
+}
+
+__global__ __launch_bounds__(256) void k_scale_y_bfp(int* __restrict__ out, const double* __restrict__ s, int nx,
+                                                     int ny, long long n)
+{
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = bfp_word(bfp_value(out[i]) * s[(i / nx) % ny]);
+}
+
+static dim3 scale_blocks(long long n)
+{
+  const long long b = (n + 255) / 256;
+  return dim3((unsigned)(b < 65536 ? b : 65536));
+}
+
+int stretch_y(x3d_ctx* c, int which, double* out)
+{
+  const long long n = (long long)c->nx * c->ny * c->nz;
+  hipLaunchKernelGGL(k_scale_y<double>, scale_blocks(n), dim3(256), 0, c->stream, out, stretch_vec<double>(c, which),
+                     c->nx, c->ny, n);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+int stretch_y_f32(x3d_ctx* c, int which, float* out)
+{
+  const long long n = (long long)c->nx * c->ny * c->nz;
+  hipLaunchKernelGGL(k_scale_y<float>, scale_blocks(n), dim3(256), 0, c->stream, out, stretch_vec<float>(c, which),
+                     c->nx, c->ny, n);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+int stretch_y_bfp(x3d_ctx* c, int which, bfp32* out)
+{
+  // a 16-lane row must hold one whole segment: nx % 16 == 0 (checked by the
+  // BFP32 entry points) and 256-thread blocks start on segment boundaries
+  const long long n = (long long)c->nx * c->ny * c->nz;
+  hipLaunchKernelGGL(k_scale_y_bfp, scale_blocks(n), dim3(256), 0, c->stream, reinterpret_cast<int*>(out),
+                     stretch_vec<double>(c, which), c->nx, c->ny, n);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+#define X3D_TRY_(expr) do { int _rc = (expr); if (_rc) return _rc; } while (0)
+
+int launch_line(x3d_ctx* c, int dir, int kind, int op, const double* in, double* out, bool periodic_fix)
+{
+  const int w = stretch_which(c, dir, kind);
+  if (w == -2) return X3D_ERR_UNSUPPORTED;
+  X3D_TRY_(line_raw(c, dir, kind, op, in, out, periodic_fix));
+  return w >= 0 ? stretch_y(c, w, out) : X3D_OK;
+}
+
+int launch_line_f32(x3d_ctx* c, int dir, int kind, int op, const float* in, float* out)
+{
+  const int w = stretch_which(c, dir, kind);
+  if (w == -2) return X3D_ERR_UNSUPPORTED;
+  X3D_TRY_(line_raw_f32(c, dir, kind, op, in, out));
+  return w >= 0 ? stretch_y_f32(c, w, out) : X3D_OK;
+}
+
+int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp32* out)
+{
+  const int w = stretch_which(c, dir, kind);
+  if (w == -2) return X3D_ERR_UNSUPPORTED;
+  X3D_TRY_(line_raw_bfp(c, dir, kind, op, in, out));
+  return w >= 0 ? stretch_y_bfp(c, w, out) : X3D_OK;
+}
+#undef X3D_TRY_
 
 }  // namespace x3d

@@ -30,6 +30,7 @@ struct TranseqArgs {
   Tiling tl;
   T gdt;
   int euler;
+  const T* sy;  // y-stretching ppy(j), or nullptr (y pass only)
 };
 
 template <typename S, int M, int DIR>
@@ -147,6 +148,10 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
     } else if constexpr (DIR == 1) {
       // tmp_c = du_c (x part) + r_c: the sum is formed here, where a strided
       // row stays inside one page, rather than in the z pass
+      if (A.sy) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) v1[k] = v1[k] * A.sy[t.j0 + k];
+      }
       if (t.active) {
 #pragma unroll
         for (int k = 0; k < M; ++k) sst(TMc, t.at(t.j0 + k), a[k] + v1[k]);
@@ -187,6 +192,7 @@ struct PassArgs {
   Geom geo;
   Tiling tl;
   int corr;
+  const T* sy;  // y-stretching: ppyi(j) for Dvp (divergence), ppy(j) for Dpv (gradp), or nullptr
 };
 
 template <typename S, int M, int DIR>
@@ -235,6 +241,10 @@ __device__ __forceinline__ void div_body(const PassArgs<S>& __restrict__ A, int 
       rhs_chunk<K_DVP, T, M, 1>(w, cfd, v);
     }
     solve_circ<T, M, DIR != 0>(v, t, ccd, lds);
+    if (A.sy) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) v[k] = v[k] * A.sy[t.j0 + k];
+    }
     {
       T w[M + 7];
       load_window<S, M, 3, 4, DIR != 0>(a, t, n, w);
@@ -303,6 +313,10 @@ __device__ __forceinline__ void grad_body(const PassArgs<S>& __restrict__ A, int
       solve_circ<T, M, DIR != 0>(v, t, cci, lds);
       store_chunk<S, M>(o1, t, v);
       solve_circ<T, M, DIR != 0>(s, t, ccd, lds);
+      if (DIR == 1 && A.sy) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) s[k] = s[k] * A.sy[t.j0 + k];
+      }
       store_chunk<S, M>(o2, t, s);
     }
     if constexpr (DIR == 1) {
@@ -512,7 +526,9 @@ static int transeq_pass(x3d_ctx* c, S* ux, S* uy, S* uz, S* dux, S* duy, S* duz,
   using T = typename Store<S>::T;
   if (!storage_ok<S>(c, DIR)) return X3D_ERR_UNSUPPORTED;
   const DirSet<S> s = dirset<S>(c, DIR);
-  const TranseqArgs<S> A{ux, uy, uz, dux, duy, duz, tx, ty, tz, s.cf_d, s.cc_d, s.geo, s.tl, (T)gdt, euler};
+  const TranseqArgs<S> A{ux,   uy,   uz,   dux,  duy,      duz,   tx,
+                         ty,   tz,   s.cf_d, s.cc_d, s.geo, s.tl, (T)gdt,
+                         euler, DIR == 1 ? stretch_vec<T>(c, 0) : nullptr};
   X3D_MSWITCH_T(S, s.M, {
     allow_lds((const void*)k_transeq<S, MM, DIR>, s.smem_tq);
     hipLaunchKernelGGL((k_transeq<S, MM, DIR>), s.grid, s.block, s.smem_tq, c->stream, A);
@@ -524,10 +540,11 @@ static int transeq_pass(x3d_ctx* c, S* ux, S* uy, S* uz, S* dux, S* duy, S* duz,
 template <int DIR, typename S>
 static int div_pass(x3d_ctx* c, const S* a, const S* b, const S* cc, S* o1, S* o2, S* o3)
 {
+  using T = typename Store<S>::T;
   if (!storage_ok<S>(c, DIR)) return X3D_ERR_UNSUPPORTED;
   const DirSet<S> s = dirset<S>(c, DIR);
   const PassArgs<S> A{a, b, cc, o1, o2, o3, nullptr, nullptr, nullptr, s.cf_dvp, s.cc_s, s.cf_ivp, s.cc_i,
-                      s.geo, s.tl, 0};
+                      s.geo, s.tl, 0, DIR == 1 ? stretch_vec<T>(c, 1) : nullptr};
   X3D_MSWITCH_T(S, s.M, hipLaunchKernelGGL((k_div<S, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
@@ -537,9 +554,11 @@ template <int DIR, typename S>
 static int grad_pass(x3d_ctx* c, const S* a, const S* b, const S* cc, S* o1, S* o2, S* o3, S* ux, S* uy, S* uz,
                      int corr)
 {
+  using T = typename Store<S>::T;
   if (!storage_ok<S>(c, DIR)) return X3D_ERR_UNSUPPORTED;
   const DirSet<S> s = dirset<S>(c, DIR);
-  const PassArgs<S> A{a, b, cc, o1, o2, o3, ux, uy, uz, s.cf_dpv, s.cc_s, s.cf_ipv, s.cc_i, s.geo, s.tl, corr};
+  const PassArgs<S> A{a,    b,    cc,       o1,     o2,       o3,     ux,    uy,   uz,
+                      s.cf_dpv, s.cc_s, s.cf_ipv, s.cc_i, s.geo, s.tl, corr, DIR == 1 ? stretch_vec<T>(c, 0) : nullptr};
   X3D_MSWITCH_T(S, s.M, hipLaunchKernelGGL((k_grad<S, MM, DIR>), s.grid, s.block, s.smem, c->stream, A));
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
@@ -619,6 +638,9 @@ constexpr int kPassDir[9] = {0, 1, 2, 0, 1, 2, 2, 1, 0};
 template <typename T>
 int batch_build(x3d_batch* b, double gdt)
 {
+  // the batched passes carry no stretching vectors
+  for (const x3d_ctx* g : b->g)
+    if (g->istret) return X3D_ERR_UNSUPPORTED;
   std::vector<unsigned char> host;
   auto put = [&host](const void* p, size_t n) {
     size_t at = (host.size() + 255) & ~size_t(255);

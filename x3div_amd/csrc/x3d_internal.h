@@ -121,6 +121,12 @@ struct x3d_ctx {
   hipStream_t cap_user = nullptr;    // the caller's stream, restored at the end
   // device copy of caller-supplied Thomas tables (f, s, w, periodic; 4 * max(nx, ny, nz))
   double* usertab = nullptr;
+  // y-stretching (istret /= 0): ppy then ppyi, ny values each, on the device
+  // in FP64 and FP32; sy_host is the host copy of the vectors last uploaded
+  int istret = 0;
+  double* sy64 = nullptr;
+  float* sy32 = nullptr;
+  std::vector<double> sy_host;
 };
 
 // A batch of independent grids (the batching form).  Each grid keeps its own
@@ -180,6 +186,31 @@ struct bfp32 {
   int32_t w;
 };
 int launch_line_bfp(x3d_ctx* c, int dir, int kind, int op, const bfp32* in, bfp32* out);
+
+// Stretched y-derivatives (src/x3d_derive.f90:331-337, 427-433,
+// src/x3d_staggered.f90:628-634, 874-880): with istret /= 0 the results of
+// dery / derypv are multiplied by ppy(j), those of deryvp by ppyi(j).
+// stretch_which: the vector a y-line operator kind takes (0 ppy, 1 ppyi), -1
+// none, -2 refused (second derivatives: a stretched grid needs pp2y/pp4y).
+inline int stretch_which(const x3d_ctx* c, int dir, int kind)
+{
+  if (!c->istret || dir != 1) return -1;
+  if (kind == K_D || kind == K_DPV) return 0;
+  if (kind == K_DVP) return 1;
+  return kind == K_DD ? -2 : -1;
+}
+// out(i, j, k) *= s(j), s = ppy (which = 0) or ppyi (which = 1)
+int stretch_y(x3d_ctx* c, int which, double* out);
+int stretch_y_f32(x3d_ctx* c, int which, float* out);
+int stretch_y_bfp(x3d_ctx* c, int which, bfp32* out);
+// device vector for the fused y passes: nullptr unless stretched
+template <typename T>
+inline const T* stretch_vec(const x3d_ctx* c, int which)
+{
+  if (!c->istret) return nullptr;
+  if constexpr (sizeof(T) == 8) return c->sy64 + (size_t)which * c->ny;
+  else return c->sy32 + (size_t)which * c->ny;
+}
 // Boundary rows of the non-periodic first derivative der?_ij
 // (src/x3d_derive.f90:156-217): codes 1 free-slip, 2 Dirichlet at each end,
 // the instance's npaire, and first_deriv_exp_'s constants for the spacing.

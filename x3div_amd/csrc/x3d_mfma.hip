@@ -215,6 +215,7 @@ extern "C" int x3d_apply_circ_mfma(x3d_ctx* c, int op, float* out, const float* 
   const int dir = op < 6 ? op / 2 : (op - 6) % 3;
   static const int kinds[4] = {K_DVP, K_DPV, K_IVP, K_IPV};
   const int kind = op < 6 ? K_D : kinds[(op - 6) / 3];
+  if (stretch_which(c, dir, kind) >= 0) return X3D_ERR_UNSUPPORTED;  // stretched y-derivatives: the line solvers
   const int n = dir == 0 ? c->nx : dir == 1 ? c->ny : c->nz;
   // the kernel's tiles: 4 wavefronts x RB blocks of 32 along the line, and
   // 32 columns (y/z) or 32 lines (x) per work-group

@@ -210,8 +210,9 @@ module variables
   integer, save, public :: nstat = 1, nvisu = 1, nprobe = 1
   logical, save, public :: test_mode = .false.
   real(mytype), save, public :: nu0nu = 0._mytype, cnu = 0._mytype
-  ! stretching vectors (istret = 0: unit), passed by dery/deryvp/derypv callers
-  real(mytype), save, allocatable, dimension(:), public :: ppy, pp2y, pp4y, ppyi, pp2yi, pp4yi
+  ! stretching vectors (src/var.f90:138-140), passed by dery/deryvp/derypv
+  ! callers; with istret /= 0 the library multiplies y-derivatives by them
+  real(mytype), save, allocatable, dimension(:), public, target :: ppy, pp2y, pp4y, ppyi, pp2yi, pp4yi
 end module variables
 
 !! Numerical parameters (module param, src/module_param.f90:79-146).

@@ -33,6 +33,11 @@ module x3d_hip
       integer(c_int), intent(in) :: ncl1(3), ncln(3)
       type(c_ptr) :: ctx
     end function
+    integer(c_int) function x3d_set_stretching(ctx, istret, ppy, ppyi) bind(C, name="x3d_set_stretching")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, ppy, ppyi
+      integer(c_int), value :: istret
+    end function
     integer(c_int) function x3d_destroy(ctx) bind(C, name="x3d_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -449,12 +449,22 @@ contains
   end subroutine derx_00
 
   subroutine dery_00(t, u, x3dop, pp, nx, ny, nz)
+    use param, only : istret
     integer, intent(in) :: nx, ny, nz
     real(mytype), intent(out), dimension(nx, ny, nz), target :: t
     real(mytype), intent(in), dimension(nx, ny, nz), target :: u
     real(mytype), intent(in), dimension(ny) :: pp
     type(x3doperator1d), intent(in) :: x3dop
-    call x3d_check('dery', x3d_dery(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), c_null_ptr, nx, ny, nz))
+    real(mytype), allocatable, target :: v(:)
+    type(c_ptr) :: pv
+    ! pp scales the result when istret /= 0 (src/x3d_derive.f90:331-337); the
+    ! DERIVATIVE_Y interface gives it no TARGET, so its values go through a copy
+    pv = c_null_ptr
+    if (istret /= 0) then
+      v = pp
+      pv = c_loc(v)
+    end if
+    call x3d_check('dery', x3d_dery(x3d_ctx_h, c_loc(t), c_loc(u), int(x3dop%id, c_int), pv, nx, ny, nz))
   end subroutine dery_00
 
   subroutine derz_00(t, u, x3dop, nx, ny, nz)

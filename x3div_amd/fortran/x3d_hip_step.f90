@@ -114,9 +114,10 @@ contains
     integer, intent(in) :: nx, ny, nym, nz
     real(mytype), intent(out), dimension(nx, nym, nz), target :: ty
     real(mytype), intent(in), dimension(nx, ny, nz), target :: uy
-    real(mytype), intent(in), dimension(nym) :: ppyi
+    real(mytype), intent(in), dimension(nym), target :: ppyi
     type(x3doperator1d), intent(in) :: x3dop
-    call x3d_check('deryvp', x3d_deryvp(x3d_ctx_h, c_loc(ty), c_loc(uy), int(x3dop%id, c_int), c_null_ptr, &
+    ! ppyi scales the result when istret /= 0 (src/x3d_staggered.f90:628-634)
+    call x3d_check('deryvp', x3d_deryvp(x3d_ctx_h, c_loc(ty), c_loc(uy), int(x3dop%id, c_int), c_loc(ppyi), &
                                         nx, ny, nym, nz))
   end subroutine deryvp
 
@@ -132,9 +133,10 @@ contains
     integer, intent(in) :: nx, nym, ny, nz
     real(mytype), intent(out), dimension(nx, ny, nz), target :: ty
     real(mytype), intent(in), dimension(nx, nym, nz), target :: uy
-    real(mytype), intent(in), dimension(ny) :: ppy
+    real(mytype), intent(in), dimension(ny), target :: ppy
     type(x3doperator1d), intent(in) :: x3dop
-    call x3d_check('derypv', x3d_derypv(x3d_ctx_h, c_loc(ty), c_loc(uy), int(x3dop%id, c_int), c_null_ptr, &
+    ! ppy scales the result when istret /= 0 (src/x3d_staggered.f90:874-880)
+    call x3d_check('derypv', x3d_derypv(x3d_ctx_h, c_loc(ty), c_loc(uy), int(x3dop%id, c_int), c_loc(ppy), &
                                         nx, nym, ny, nz))
   end subroutine derypv
 
@@ -408,9 +410,15 @@ contains
   subroutine var_init()
     use decomp_2d, only : xsize, ph1
     use param, only : adt, bdt, gdt, dt, ntime, npress, zero, one
-    use variables, only : nzm
+    use variables, only : ny, nzm, ppy, pp2y, pp4y, ppyi, pp2yi, pp4yi
     integer :: i
     integer(c_size_t) :: nv, np
+    ! stretching vectors, zero as at the reference (src/var.f90:138-140)
+    if (.not. allocated(ppy)) then
+      allocate (ppy(ny))
+      ppy = zero
+      allocate (pp2y, pp4y, ppyi, pp2yi, pp4yi, source=ppy)
+    end if
     adt = zero; bdt = zero; gdt = zero
     adt(1) = one * dt
     gdt(1) = adt(1) + bdt(1)
@@ -435,7 +443,9 @@ contains
   end subroutine var_init
 
   subroutine var_finalize()
+    use variables, only : ppy, pp2y, pp4y, ppyi, pp2yi, pp4yi
     integer :: i
+    if (allocated(ppy)) deallocate (ppy, pp2y, pp4y, ppyi, pp2yi, pp4yi)
     do i = 1, 10
       if (c_associated(mem(i))) call x3d_check('x3d_field_free', x3d_field_free(x3d_ctx_h, mem(i)))
       mem(i) = c_null_ptr

@@ -18,11 +18,11 @@ subroutine init_xcompact3d(ndt_max)
   use decomp_2d_poisson, only : decomp_2d_poisson_init
   use x3d_operator_1d, only : x3d_operator_1d_init
   use x3d_derive, only : x3d_derive_init
-  use x3d_hip, only : x3d_ctx_h, x3d_check, x3d_create_bc
+  use x3d_hip, only : x3d_ctx_h, x3d_check, x3d_create_bc, x3d_set_stretching
   use parameters, only : parameter, listing
-  use param, only : xlx, yly, zlz, nclx1, nclxn, ncly1, nclyn, nclz1, nclzn
+  use param, only : xlx, yly, zlz, nclx1, nclxn, ncly1, nclyn, nclz1, nclzn, istret
   use var, only : var_init
-  use variables, only : nx, ny, nz, nxm, nym, nzm, p_row, p_col, test_mode
+  use variables, only : nx, ny, nz, nxm, nym, nzm, p_row, p_col, test_mode, ppy, ppyi
   implicit none
   integer, intent(inout) :: ndt_max
   integer :: nargin, arg, flen, status, decind
@@ -74,6 +74,10 @@ subroutine init_xcompact3d(ndt_max)
                                              int([nclx1, ncly1, nclz1], c_int), int([nclxn, nclyn, nclzn], c_int), &
                                              x3d_ctx_h))
   call var_init()
+  ! a stretched y grid: the step's y-derivatives take ppy / ppyi (istret = 0
+  ! in this driver, src/parameters.f90:117)
+  if (istret /= 0) call x3d_check('x3d_set_stretching', x3d_set_stretching(x3d_ctx_h, int(istret, c_int), &
+                                                                            c_loc(ppy), c_loc(ppyi)))
   call x3d_operator_1d_init()
   call x3d_derive_init()
 
