@@ -1,5 +1,6 @@
-"""Poisson with the fused z stage (2-D hipFFT over x, y and one kernel for the
-forward z DFT, the division and the inverse z DFT) against the oracle, for
+"""Poisson with the fused z stage (the x r2c and forward y DFT as line
+kernels, or hipFFT's 2-D plan, then one kernel for the forward z DFT, the
+division and the inverse z DFT) against the oracle, for
 every z length the kernel is built for (radix-8 passes with a final radix-4
 or radix-2 pass, then radix-3 and radix-5 passes for 3- and 5-smooth
 lengths), in both precisions.  FP64 at the 1e-12 bar,
@@ -72,6 +73,41 @@ def test_poisson_fused_z_y_lengths(torch_cuda, ny):
     e64 = np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max()
     e32 = np.abs(t32.cpu().numpy().astype(np.float64) - ref32).max() / np.abs(ref32).max()
     print(f"ny={ny}: fp64 {e64:.2e} fp32 {e32:.2e}")
+    ctx.close()
+    assert e64 < 1e-12, e64
+    assert e32 < 2e-6, e32
+
+
+# nx = 2M with M a line-kernel length: the forward x r2c runs as the x-line
+# kernel (one length-M complex DFT per line and the split into the half
+# spectrum), then the forward y-line kernel; other nx keep hipFFT's 2-D plan
+NX = [16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 768, 1024, 24, 20]
+
+
+@pytest.mark.parametrize("nx", NX)
+@pytest.mark.parametrize("xsplit", ["1", "0"])
+def test_poisson_x_lengths(torch_cuda, nx, xsplit, monkeypatch):
+    """Every x length of the x r2c line kernel, with and without it
+    (X3D_XSPLIT=0: hipFFT's 2-D forward plan), both precisions."""
+    import torch
+    from x3div_amd import X3D
+    monkeypatch.setenv("X3D_XSPLIT", xsplit)
+    ny, nz = (24, 16) if nx <= 256 else (16, 8)
+    print(f"\n[poisson x {nx}x{ny}x{nz} xsplit={xsplit}]", flush=True)
+    ctx = X3D(nx, ny, nz, 3.0, 1.0, 2.0)
+    orc = Oracle(nx, ny, nz, 3.0, 1.0, 2.0)
+    rhs = np.random.default_rng(nx).standard_normal(orc.shape)
+    ref = orc.poisson(rhs)
+    t = torch.from_numpy(rhs.copy()).cuda()
+    ctx.poisson(t)
+    r32 = np.asarray(rhs, dtype=np.float32)
+    ref32 = orc.poisson(r32.astype(np.float64))
+    t32 = torch.from_numpy(r32.copy()).cuda()
+    ctx.poisson_f32(t32)
+    torch.cuda.synchronize()
+    e64 = np.abs(t.cpu().numpy() - ref).max() / np.abs(ref).max()
+    e32 = np.abs(t32.cpu().numpy().astype(np.float64) - ref32).max() / np.abs(ref32).max()
+    print(f"nx={nx}: fp64 {e64:.2e} fp32 {e32:.2e}")
     ctx.close()
     assert e64 < 1e-12, e64
     assert e32 < 2e-6, e32

@@ -98,9 +98,11 @@ struct x3d_ctx {
   void* ztw = nullptr;              // e^{-2 pi i t / nz}, t < nz (double2)
   bool ysplit = false;              // and x-only hipFFT plans + y stage kernels (ny a power of two <= 512)
   void* ytw = nullptr;              // e^{-2 pi i t / ny}, t < ny (double2)
+  bool xsplit = false;              // and the forward x r2c and y DFT as line kernels (X3D_XSPLIT=0: hipFFT 2-D)
+  void* xtw = nullptr;              // e^{-2 pi i t / nx}, t < nx (double2)
   int ncu = 0;                      // compute units of the context's device
   int xcd_remap = 1;                // line blocks regrouped per XCD (X3D_XCD_REMAP=0 turns it off)
-  int spl_wg[2][2] = {{0, 0}, {0, 0}};  // resident work-groups/CU of the Poisson line kernels [FP64/FP32][z/y]
+  int spl_wg[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident work-groups/CU of the Poisson line kernels [FP64/FP32][mode]
   std::vector<double> ax, bx, ay, by, az, bz;
   std::vector<double> xk2, yk2, zk2, exs, eys, ezs;
   // step scratch fields (allocated on first use)

@@ -338,9 +338,9 @@ __global__ __launch_bounds__(NT) void k_poisson_z(C* __restrict__ spec, const do
 //   MODE 0 (z-lines): forward DFT, division, inverse DFT; the expressions of
 //          k_poisson_z in the same order (products may fuse into
 //          multiply-adds differently, so results agree to rounding).
-//   MODE 2 (y-lines): the inverse length-ny DFT alone, for the
-//          y stage of the 3-D transform.  A tile is TI x-columns of one
-//          z-plane, so its rows share the plane's pages.
+//   MODE 1 / 2 (y-lines): the forward / inverse length-ny DFT alone, for
+//          the y stages of the 3-D transform.  A tile is TI x-columns of
+//          one z-plane, so its rows share the plane's pages.
 // Tile t covers columns i0.. of outer line index o (y for MODE 0, z else).
 template <typename C, int N, int TI, int MODE>
 constexpr size_t spl_lds_bytes()
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
 {
 #pragma clang fp contract(off)
   static_assert(NT % TI == 0, "a thread keeps one column");
-  static_assert(MODE == 0 || MODE == 2, "z stage or inverse y stage");
+  static_assert(MODE >= 0 && MODE <= 2, "z stage, forward or inverse y stage");
   constexpr int NE = N * TI;
   constexpr int E = (NE + NT - 1) / NT;  // elements per thread
   constexpr int NTW = N / 2 > 0 ? N / 2 : 1;
@@ -365,9 +365,9 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
   // its first butterfly, and the outputs of its last forward butterfly are the
   // inputs of its first inverse one and, after the inverse, the points it
   // stores.  Those passes skip LDS, and the division is applied in registers.
-  // (FP64 y-lines excepted: with only the inverse transform after the
-  // prefetch, which here starts one pass later, too little work hides it)
-  constexpr bool REG = N == 512 && TI * (N / 8) == NT && !(MODE == 2 && sizeof(C) == 16);
+  // (FP64 y-lines excepted: with only one transform after the prefetch,
+  // which here starts one pass later, too little work hides it)
+  constexpr bool REG = N == 512 && TI * (N / 8) == NT && !(MODE != 0 && sizeof(C) == 16);
   extern __shared__ unsigned char smem_raw[];
   C* buf = reinterpret_cast<C*>(smem_raw);
   double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * NE);
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
       C v[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = P[r];  // points jj + 64 r
-      constexpr bool inv0 = MODE == 2;  // z: forward first; y: inverse only
+      constexpr bool inv0 = MODE == 2;  // z: forward first; y: one direction only
       stockham_bfly<C, N, 8>(v, twl, jj, 1, inv0);
 #pragma unroll
       for (int r = 0; r < 8; ++r) buf[(8 * jj + r) * TI + col] = v[r];
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
       }
       __syncthreads();
     }
-    line_dft<C, N, TI, NT>(buf, twl, true);
+    line_dft<C, N, TI, NT>(buf, twl, MODE != 1);
 #pragma unroll 1
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q;
@@ -528,6 +528,59 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
     }
     // buf is refilled by the next tile
     __syncthreads();
+  }
+}
+
+// Forward x transform (r2c) of every x-line: a real line of nx = 2M points
+// is read as M complex points z_j = x_{2j} + i x_{2j+1} (its own memory,
+// one 2-word load per point), one length-M DFT gives Z, and the half
+// spectrum is X_k = E_k - i W^k O_k, k = 0..M, with
+// E_k = (Z_k + conj Z_{M-k}) / 2, O_k = (Z_k - conj Z_{M-k}) / 2,
+// W = e^{-2 pi i / nx} and Z_M = Z_0.  A work-group takes TI consecutive
+// lines; a wavefront's loads and stores cover 64 / TI points of each of its
+// lines (lanes: line fastest), so that the LDS accesses of the column
+// layout buf[point * TI + line] stay free of bank conflicts.
+template <typename C, int M, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_rfft_x(const typename CplxT<C>::R* __restrict__ rhs, C* __restrict__ spec,
+                                               const double2* __restrict__ tw, long long nlines)
+{
+#pragma clang fp contract(off)
+  using R = typename CplxT<C>::R;
+  constexpr int NE = M * TI;
+  constexpr int NTW = M / 2 > 0 ? M / 2 : 1;
+  constexpr int NH = M + 1;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf = reinterpret_cast<C*>(smem_raw);
+  double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * NE);  // e^{-2 pi i t / M}, t < M / 2
+  double2* wn = twl + NTW;                                                // W^k, k < M
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NTW; e += NT) twl[e] = tw[2 * e];
+  for (int e = tid; e < M; e += NT) wn[e] = tw[e];
+  const long long l0 = (long long)blockIdx.x * TI;
+  const C* in = reinterpret_cast<const C*>(rhs);
+  for (int e = tid; e < NE; e += NT) {
+    const int lt = e % TI, j = e / TI;
+    C v;
+    v.x = 0;
+    v.y = 0;
+    if (l0 + lt < nlines) v = in[(l0 + lt) * M + j];
+    buf[e] = v;  // = buf[j * TI + lt]
+  }
+  __syncthreads();
+  line_dft<C, M, TI, NT>(buf, twl, false);
+  for (int e = tid; e < NH * TI; e += NT) {
+    const int lt = e % TI, k = e / TI;
+    if (l0 + lt >= nlines) continue;
+    const C a = buf[(k == M ? 0 : k) * TI + lt];
+    const C b = buf[(k == 0 ? 0 : M - k) * TI + lt];  // Z_{M-k}, unconjugated
+    const R ex = (a.x + b.x) * (R)0.5, ey = (a.y - b.y) * (R)0.5;
+    const R ox = (a.x - b.x) * (R)0.5, oy = (a.y + b.y) * (R)0.5;
+    const R wx = k == M ? (R)-1 : (R)wn[k].x, wy = k == M ? (R)0 : (R)wn[k].y;
+    const R txx = wx * ox - wy * oy, tyy = wx * oy + wy * ox;  // W^k O_k
+    C x;
+    x.x = ex + tyy;
+    x.y = ey - txx;
+    spec[(l0 + lt) * NH + k] = x;
   }
 }
 
@@ -559,11 +612,11 @@ constexpr bool spl_ok()
 }
 
 // Slot of a line kernel's resident work-groups per compute unit in the
-// context: [FP64 / FP32][z stage / inverse y stage].
+// context: [FP64 / FP32][z stage / forward y stage / inverse y stage].
 template <typename C, int MODE>
 static int& spl_slot(x3d_ctx* c)
 {
-  return c->spl_wg[sizeof(C) == 16 ? 0 : 1][MODE == 0 ? 0 : 1];
+  return c->spl_wg[sizeof(C) == 16 ? 0 : 1][MODE];
 }
 
 // At context set-up, on the context's device: raise the line kernel's LDS
@@ -638,28 +691,69 @@ static int launch_pz_n(x3d_ctx* c, C* spec)
   return X3D_OK;
 }
 
-// inverse y stage of the 3-D transform, for the y
+// y stages of the 3-D transform (MODE 1 forward, 2 inverse), for the y
 // lengths ysplit_ok admits
-template <typename C>
+template <typename C, int MODE>
 static int launch_poisson_y(x3d_ctx* c, C* spec)
 {
   switch (c->ny) {
-    case 8: return launch_spl<C, 8, 2>(c, spec, c->ytw);
-    case 16: return launch_spl<C, 16, 2>(c, spec, c->ytw);
-    case 32: return launch_spl<C, 32, 2>(c, spec, c->ytw);
-    case 64: return launch_spl<C, 64, 2>(c, spec, c->ytw);
-    case 128: return launch_spl<C, 128, 2>(c, spec, c->ytw);
-    case 256: return launch_spl<C, 256, 2>(c, spec, c->ytw);
-    case 512: return launch_spl<C, 512, 2>(c, spec, c->ytw);
-    case 24: return launch_spl<C, 24, 2>(c, spec, c->ytw);
-    case 40: return launch_spl<C, 40, 2>(c, spec, c->ytw);
-    case 48: return launch_spl<C, 48, 2>(c, spec, c->ytw);
-    case 80: return launch_spl<C, 80, 2>(c, spec, c->ytw);
-    case 96: return launch_spl<C, 96, 2>(c, spec, c->ytw);
-    case 160: return launch_spl<C, 160, 2>(c, spec, c->ytw);
-    case 192: return launch_spl<C, 192, 2>(c, spec, c->ytw);
-    case 320: return launch_spl<C, 320, 2>(c, spec, c->ytw);
-    case 384: return launch_spl<C, 384, 2>(c, spec, c->ytw);
+    case 8: return launch_spl<C, 8, MODE>(c, spec, c->ytw);
+    case 16: return launch_spl<C, 16, MODE>(c, spec, c->ytw);
+    case 32: return launch_spl<C, 32, MODE>(c, spec, c->ytw);
+    case 64: return launch_spl<C, 64, MODE>(c, spec, c->ytw);
+    case 128: return launch_spl<C, 128, MODE>(c, spec, c->ytw);
+    case 256: return launch_spl<C, 256, MODE>(c, spec, c->ytw);
+    case 512: return launch_spl<C, 512, MODE>(c, spec, c->ytw);
+    case 24: return launch_spl<C, 24, MODE>(c, spec, c->ytw);
+    case 40: return launch_spl<C, 40, MODE>(c, spec, c->ytw);
+    case 48: return launch_spl<C, 48, MODE>(c, spec, c->ytw);
+    case 80: return launch_spl<C, 80, MODE>(c, spec, c->ytw);
+    case 96: return launch_spl<C, 96, MODE>(c, spec, c->ytw);
+    case 160: return launch_spl<C, 160, MODE>(c, spec, c->ytw);
+    case 192: return launch_spl<C, 192, MODE>(c, spec, c->ytw);
+    case 320: return launch_spl<C, 320, MODE>(c, spec, c->ytw);
+    case 384: return launch_spl<C, 384, MODE>(c, spec, c->ytw);
+    default: return X3D_ERR_UNSUPPORTED;
+  }
+}
+
+// The x r2c kernel for nx = 2M: LDS of its tile and tables, and the set-up
+// call that raises its LDS limit (outside any capture).
+template <typename C, int M>
+constexpr size_t rfx_lds_bytes()
+{
+  return (size_t)M * LineTile<C, M>::TI * sizeof(C) + (size_t)((M / 2 > 0 ? M / 2 : 1) + M) * sizeof(double2);
+}
+
+template <typename C, int M>
+static int rfx_run(x3d_ctx* c, const typename CplxT<C>::R* rhs, C* spec, bool prepare)
+{
+  constexpr int TI = LineTile<C, M>::TI, NT = LineTile<C, M>::NT;
+  constexpr size_t smem = rfx_lds_bytes<C, M>();
+  static_assert(smem <= 160 * 1024, "x r2c tile fits the LDS");
+  const void* fn = (const void*)k_rfft_x<C, M, TI, NT>;
+  if (prepare) {
+    if (smem > 65536) X3D_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    return X3D_OK;
+  }
+  const long long nlines = (long long)c->ny * c->nz;
+  const long long blocks = (nlines + TI - 1) / TI;
+  hipLaunchKernelGGL((k_rfft_x<C, M, TI, NT>), dim3((unsigned)blocks), dim3(NT), smem, c->stream, rhs, spec,
+                     reinterpret_cast<const double2*>(c->xtw), nlines);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
+// nx = 2M with M one of the line kernels' lengths up to 512
+template <typename C>
+static int launch_rfft_x(x3d_ctx* c, const typename CplxT<C>::R* rhs, C* spec, bool prepare = false)
+{
+  switch (c->nx / 2) {
+#define X3D_RFX(MM) case MM: return rfx_run<C, MM>(c, rhs, spec, prepare);
+    X3D_RFX(8) X3D_RFX(16) X3D_RFX(32) X3D_RFX(64) X3D_RFX(128) X3D_RFX(256) X3D_RFX(512)
+    X3D_RFX(24) X3D_RFX(40) X3D_RFX(48) X3D_RFX(80) X3D_RFX(96) X3D_RFX(160) X3D_RFX(192) X3D_RFX(320)
+    X3D_RFX(384)
+#undef X3D_RFX
     default: return X3D_ERR_UNSUPPORTED;
   }
 }
@@ -705,6 +799,7 @@ static bool smooth_len(int n)
 }
 static bool zsplit_ok(int nz) { return nz >= 8 && nz <= 1024 && smooth_len(nz); }
 static bool ysplit_ok(int ny) { return ny >= 8 && ny <= 512 && smooth_len(ny); }
+static bool xsplit_ok(int nx) { return nx % 2 == 0 && nx / 2 >= 8 && nx / 2 <= 512 && smooth_len(nx / 2); }
 
 // Table of e^{-2 pi i t / n}, t < n, on the device.
 static int twiddles(int n, void** dev)
@@ -798,12 +893,21 @@ int poisson_setup(x3d_ctx* c)
   c->zsplit = zsplit_ok(c->nz);
   // (the y kernel's 32-bit offsets span 64 rows of the half spectrum)
   c->ysplit = c->zsplit && ysplit_ok(c->ny) && (long long)(c->nx / 2 + 1) * 64 * 16 < (1LL << 32);
+  // the forward x r2c as a line kernel as well (its 32-bit-free indexing
+  // takes any size; X3D_XSPLIT=0 keeps hipFFT's 2-D forward plan)
+  const char* xs = std::getenv("X3D_XSPLIT");
+  c->xsplit = c->ysplit && xsplit_ok(c->nx) && !(xs && std::atoi(xs) == 0);
   if (c->zsplit) X3D_TRY_RC(twiddles(c->nz, &c->ztw));
   if (c->ysplit) X3D_TRY_RC(twiddles(c->ny, &c->ytw));
+  if (c->xsplit) X3D_TRY_RC(twiddles(c->nx, &c->xtw));
   if (!make_plans(c, &c->fft_r2c, &c->fft_c2r, HIPFFT_D2Z, HIPFFT_Z2D)) return X3D_ERR_FFT;
   X3D_CHECK_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device));
   if (c->zsplit) X3D_TRY_RC((spl_prepare<double2, 0>(c, c->nz)));
   if (c->ysplit) X3D_TRY_RC((spl_prepare<double2, 2>(c, c->ny)));
+  if (c->xsplit) {
+    X3D_TRY_RC((spl_prepare<double2, 1>(c, c->ny)));
+    X3D_TRY_RC(launch_rfft_x<double2>(c, nullptr, nullptr, true));
+  }
   c->fft_ready = true;
   if (hipfftSetStream(c->fft_r2c, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   if (hipfftSetStream(c->fft_c2r, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
@@ -826,10 +930,12 @@ void poisson_release(x3d_ctx* c)
   if (c->ktab) (void)hipFree(c->ktab);
   if (c->ztw) (void)hipFree(c->ztw);
   if (c->ytw) (void)hipFree(c->ytw);
+  if (c->xtw) (void)hipFree(c->xtw);
   c->spec = nullptr;
   c->ktab = nullptr;
   c->ztw = nullptr;
   c->ytw = nullptr;
+  c->xtw = nullptr;
 }
 
 // FP32 spectrum, FP64 division: the modified wavenumber and the quotient are
@@ -874,6 +980,10 @@ int poisson_prepare_f32(x3d_ctx* c)
     if (!make_plans(c, &c->fft_r2c32, &c->fft_c2r32, HIPFFT_R2C, HIPFFT_C2R)) return X3D_ERR_FFT;
     if (c->zsplit) X3D_TRY_RC((spl_prepare<float2, 0>(c, c->nz)));
     if (c->ysplit) X3D_TRY_RC((spl_prepare<float2, 2>(c, c->ny)));
+    if (c->xsplit) {
+      X3D_TRY_RC((spl_prepare<float2, 1>(c, c->ny)));
+      X3D_TRY_RC(launch_rfft_x<float2>(c, nullptr, nullptr, true));
+    }
     c->fft32_ready = true;
   }
   return X3D_OK;
@@ -887,10 +997,15 @@ int poisson_solve_f32(x3d_ctx* c, float* rhs)
   if (hipfftSetStream(c->fft_c2r32, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   // the FP64 spectrum buffer holds the FP32 one
   float2* spec = reinterpret_cast<float2*>(c->spec);
-  if (hipfftExecR2C(c->fft_r2c32, rhs, (hipfftComplex*)spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  if (c->xsplit) {
+    X3D_TRY_RC(launch_rfft_x<float2>(c, rhs, spec));
+    X3D_TRY_RC((launch_poisson_y<float2, 1>(c, spec)));
+  } else if (hipfftExecR2C(c->fft_r2c32, rhs, (hipfftComplex*)spec) != HIPFFT_SUCCESS) {
+    return X3D_ERR_FFT;
+  }
   if (c->zsplit) {
     X3D_TRY_RC(launch_poisson_z<float2>(c, spec));
-    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<float2>(c, spec)));
+    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<float2, 2>(c, spec)));
     if (hipfftExecC2R(c->fft_c2r32, (hipfftComplex*)spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
     return X3D_OK;
   }
@@ -908,10 +1023,15 @@ int poisson_solve_f32(x3d_ctx* c, float* rhs)
 int poisson_solve(x3d_ctx* c, double* rhs)
 {
   if (!c->fft_ready) return X3D_ERR_FFT;
-  if (hipfftExecD2Z(c->fft_r2c, rhs, (hipfftDoubleComplex*)c->spec) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
+  if (c->xsplit) {
+    X3D_TRY_RC(launch_rfft_x<double2>(c, rhs, c->spec));
+    X3D_TRY_RC((launch_poisson_y<double2, 1>(c, c->spec)));
+  } else if (hipfftExecD2Z(c->fft_r2c, rhs, (hipfftDoubleComplex*)c->spec) != HIPFFT_SUCCESS) {
+    return X3D_ERR_FFT;
+  }
   if (c->zsplit) {
     X3D_TRY_RC(launch_poisson_z<double2>(c, c->spec));
-    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<double2>(c, c->spec)));
+    if (c->ysplit) X3D_TRY_RC((launch_poisson_y<double2, 2>(c, c->spec)));
     if (hipfftExecZ2D(c->fft_c2r, (hipfftDoubleComplex*)c->spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
     return X3D_OK;
   }
