@@ -197,9 +197,9 @@ __device__ __forceinline__ void stockham_bfly(C (&v)[R], const double2* __restri
   small_dft<C, R>(v, inv);
 }
 
-// One Stockham pass of radix R over the TI lines held in buf[k][TI]
-// (Ns = product of the radices already applied).
-template <typename C, int N, int R, int TI, int NT>
+// One Stockham pass of radix R over the TI lines held in buf[k][ST] (row
+// stride ST >= TI elements; Ns = product of the radices already applied).
+template <typename C, int N, int R, int TI, int NT, int ST = TI>
 __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict__ tw, int Ns, bool inv)
 {
   constexpr int NB = TI * N / R;             // butterflies in the work-group
@@ -212,7 +212,7 @@ __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict_
     if (b < NB) {
       const int col = b % TI, jj = b / TI;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = buf[(jj + r * (N / R)) * TI + col];
+      for (int r = 0; r < R; ++r) v[q][r] = buf[(jj + r * (N / R)) * ST + col];
     }
   }
   __syncthreads();
@@ -224,7 +224,7 @@ __device__ __forceinline__ void stockham_pass(C* buf, const double2* __restrict_
       stockham_bfly<C, N, R>(v[q], tw, jj, Ns, inv);
       const int d = (jj / Ns) * Ns * R + jj % Ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(d + r * Ns) * TI + col] = v[q][r];
+      for (int r = 0; r < R; ++r) buf[(d + r * Ns) * ST + col] = v[q][r];
     }
   }
   __syncthreads();
@@ -237,7 +237,7 @@ constexpr int pow_part(int n, int p) { return n % p ? 1 : p * pow_part(n / p, p)
 constexpr int count_factor(int n, int p) { return n % p ? 0 : 1 + count_factor(n / p, p); }
 constexpr bool smooth235(int n) { return n == pow_part(n, 2) * pow_part(n, 3) * pow_part(n, 5); }
 
-template <typename C, int N, int TI, int NT>
+template <typename C, int N, int TI, int NT, int ST = TI>
 __device__ __forceinline__ void line_dft(C* buf, const double2* __restrict__ tw, bool inv)
 {
   static_assert(smooth235(N), "line lengths are 2^a 3^b 5^c");
@@ -245,25 +245,25 @@ __device__ __forceinline__ void line_dft(C* buf, const double2* __restrict__ tw,
   constexpr int L = __builtin_ctz(N), L3 = count_factor(N, 3), L5 = count_factor(N, 5);
 #pragma unroll
   for (int p = 0; p < L / 3; ++p) {
-    stockham_pass<C, N, 8, TI, NT>(buf, tw, Ns, inv);
+    stockham_pass<C, N, 8, TI, NT, ST>(buf, tw, Ns, inv);
     Ns *= 8;
   }
   if constexpr (L % 3 == 2) {
-    stockham_pass<C, N, 4, TI, NT>(buf, tw, Ns, inv);
+    stockham_pass<C, N, 4, TI, NT, ST>(buf, tw, Ns, inv);
     Ns *= 4;
   }
   if constexpr (L % 3 == 1) {
-    stockham_pass<C, N, 2, TI, NT>(buf, tw, Ns, inv);
+    stockham_pass<C, N, 2, TI, NT, ST>(buf, tw, Ns, inv);
     Ns *= 2;
   }
 #pragma unroll
   for (int p = 0; p < L3; ++p) {
-    stockham_pass<C, N, 3, TI, NT>(buf, tw, Ns, inv);
+    stockham_pass<C, N, 3, TI, NT, ST>(buf, tw, Ns, inv);
     Ns *= 3;
   }
 #pragma unroll
   for (int p = 0; p < L5; ++p) {
-    stockham_pass<C, N, 5, TI, NT>(buf, tw, Ns, inv);
+    stockham_pass<C, N, 5, TI, NT, ST>(buf, tw, Ns, inv);
     Ns *= 5;
   }
 }
@@ -536,51 +536,73 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
 // one 2-word load per point), one length-M DFT gives Z, and the half
 // spectrum is X_k = E_k - i W^k O_k, k = 0..M, with
 // E_k = (Z_k + conj Z_{M-k}) / 2, O_k = (Z_k - conj Z_{M-k}) / 2,
-// W = e^{-2 pi i / nx} and Z_M = Z_0.  A work-group takes TI consecutive
-// lines; a wavefront's loads and stores cover 64 / TI points of each of its
-// lines (lanes: line fastest), so that the LDS accesses of the column
-// layout buf[point * TI + line] stay free of bank conflicts.
+// W = e^{-2 pi i / nx} and Z_M = Z_0.  A tile is TI consecutive lines; each
+// work-group copies the twiddles to LDS once and walks tiles blockIdx.x,
+// +gridDim.x, ...  Loads and stores run along the lines (coalesced), and the
+// column layout in LDS, buf[point * ST + line], has the odd row stride
+// ST = TI + 1 so that those accesses spread over the banks.
 template <typename C, int M, int TI, int NT>
 __global__ __launch_bounds__(NT) void k_rfft_x(const typename CplxT<C>::R* __restrict__ rhs, C* __restrict__ spec,
                                                const double2* __restrict__ tw, long long nlines)
 {
 #pragma clang fp contract(off)
   using R = typename CplxT<C>::R;
+  constexpr int ST = TI + 1;
   constexpr int NE = M * TI;
   constexpr int NTW = M / 2 > 0 ? M / 2 : 1;
   constexpr int NH = M + 1;
   extern __shared__ unsigned char smem_raw[];
   C* buf = reinterpret_cast<C*>(smem_raw);
-  double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * NE);  // e^{-2 pi i t / M}, t < M / 2
-  double2* wn = twl + NTW;                                                // W^k, k < M
+  double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * M * ST);  // e^{-2 pi i t / M}, t < M / 2
+  double2* wn = twl + NTW;                                                   // W^k, k < M
   const int tid = threadIdx.x;
   for (int e = tid; e < NTW; e += NT) twl[e] = tw[2 * e];
   for (int e = tid; e < M; e += NT) wn[e] = tw[e];
-  const long long l0 = (long long)blockIdx.x * TI;
   const C* in = reinterpret_cast<const C*>(rhs);
-  for (int e = tid; e < NE; e += NT) {
-    const int lt = e % TI, j = e / TI;
-    C v;
-    v.x = 0;
-    v.y = 0;
-    if (l0 + lt < nlines) v = in[(l0 + lt) * M + j];
-    buf[e] = v;  // = buf[j * TI + lt]
-  }
-  __syncthreads();
-  line_dft<C, M, TI, NT>(buf, twl, false);
-  for (int e = tid; e < NH * TI; e += NT) {
-    const int lt = e % TI, k = e / TI;
-    if (l0 + lt >= nlines) continue;
-    const C a = buf[(k == M ? 0 : k) * TI + lt];
-    const C b = buf[(k == 0 ? 0 : M - k) * TI + lt];  // Z_{M-k}, unconjugated
-    const R ex = (a.x + b.x) * (R)0.5, ey = (a.y - b.y) * (R)0.5;
-    const R ox = (a.x - b.x) * (R)0.5, oy = (a.y + b.y) * (R)0.5;
-    const R wx = k == M ? (R)-1 : (R)wn[k].x, wy = k == M ? (R)0 : (R)wn[k].y;
-    const R txx = wx * ox - wy * oy, tyy = wx * oy + wy * ox;  // W^k O_k
-    C x;
-    x.x = ex + tyy;
-    x.y = ey - txx;
-    spec[(l0 + lt) * NH + k] = x;
+  const long long ntiles = (nlines + TI - 1) / TI;
+  // the next tile's points are loaded into registers while this one is
+  // transformed (the transforms read LDS only, so they do not wait on them)
+  constexpr int E = (NE + NT - 1) / NT;
+  C P[E];
+  auto fetch = [&](long long tile) {
+    const long long l0 = tile * TI;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int e = tid + NT * q, lt = e / M, j = e - lt * M;
+      P[q].x = 0;
+      P[q].y = 0;
+      if (e < NE && l0 + lt < nlines) P[q] = in[(l0 + lt) * M + j];
+    }
+  };
+  long long tile = blockIdx.x;
+  if (tile < ntiles) fetch(tile);
+  for (; tile < ntiles; tile += gridDim.x) {
+    const long long l0 = tile * TI;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int e = tid + NT * q, lt = e / M, j = e - lt * M;
+      if (e < NE) buf[j * ST + lt] = P[q];
+    }
+    // (the first barrier also orders the twiddle copy before its readers)
+    __syncthreads();
+    if (tile + (long long)gridDim.x < ntiles) fetch(tile + gridDim.x);
+    line_dft<C, M, TI, NT, ST>(buf, twl, false);
+    for (int e = tid; e < NH * TI; e += NT) {
+      const int lt = e / NH, k = e - lt * NH;
+      if (l0 + lt >= nlines) continue;
+      const C a = buf[(k == M ? 0 : k) * ST + lt];
+      const C b = buf[(k == 0 ? 0 : M - k) * ST + lt];  // Z_{M-k}, unconjugated
+      const R ex = (a.x + b.x) * (R)0.5, ey = (a.y - b.y) * (R)0.5;
+      const R ox = (a.x - b.x) * (R)0.5, oy = (a.y + b.y) * (R)0.5;
+      const R wx = k == M ? (R)-1 : (R)wn[k].x, wy = k == M ? (R)0 : (R)wn[k].y;
+      const R txx = wx * ox - wy * oy, tyy = wx * oy + wy * ox;  // W^k O_k
+      C x;
+      x.x = ex + tyy;
+      x.y = ey - txx;
+      spec[(l0 + lt) * NH + k] = x;
+    }
+    // buf is refilled by the next tile
+    __syncthreads();
   }
 }
 
@@ -717,27 +739,42 @@ static int launch_poisson_y(x3d_ctx* c, C* spec)
   }
 }
 
-// The x r2c kernel for nx = 2M: LDS of its tile and tables, and the set-up
-// call that raises its LDS limit (outside any capture).
+// The x r2c kernel for nx = 2M: its tile (16 lines; 1024 threads from 128
+// points on, which fills a compute unit with two FP64 or FP32 work-groups),
+// the LDS of the tile and tables, and the set-up call that raises its LDS
+// limit (outside any capture).
+template <typename C, int M>
+struct RfxTile {
+  static constexpr int TI = LineTile<C, M>::TI;
+  static constexpr int NT = M >= 128 ? 1024 : LineTile<C, M>::NT;
+};
 template <typename C, int M>
 constexpr size_t rfx_lds_bytes()
 {
-  return (size_t)M * LineTile<C, M>::TI * sizeof(C) + (size_t)((M / 2 > 0 ? M / 2 : 1) + M) * sizeof(double2);
+  return (size_t)M * (RfxTile<C, M>::TI + 1) * sizeof(C) + (size_t)((M / 2 > 0 ? M / 2 : 1) + M) * sizeof(double2);
 }
 
 template <typename C, int M>
 static int rfx_run(x3d_ctx* c, const typename CplxT<C>::R* rhs, C* spec, bool prepare)
 {
-  constexpr int TI = LineTile<C, M>::TI, NT = LineTile<C, M>::NT;
+  constexpr int TI = RfxTile<C, M>::TI, NT = RfxTile<C, M>::NT;
   constexpr size_t smem = rfx_lds_bytes<C, M>();
   static_assert(smem <= 160 * 1024, "x r2c tile fits the LDS");
   const void* fn = (const void*)k_rfft_x<C, M, TI, NT>;
+  int& per_cu = c->rfx_wg[sizeof(C) == 16 ? 0 : 1];
   if (prepare) {
     if (smem > 65536) X3D_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    int nb = 0;
+    X3D_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, smem));
+    per_cu = nb > 0 ? nb : 1;
     return X3D_OK;
   }
+  if (per_cu <= 0) return X3D_ERR_HIP;  // set-up did not prepare this kernel
   const long long nlines = (long long)c->ny * c->nz;
-  const long long blocks = (nlines + TI - 1) / TI;
+  const long long ntiles = (nlines + TI - 1) / TI;
+  // one launch wave of work-groups: compute units x resident work-groups each
+  const long long cap = (long long)(c->ncu > 0 ? c->ncu : 1) * per_cu;
+  const long long blocks = ntiles < cap ? ntiles : cap;
   hipLaunchKernelGGL((k_rfft_x<C, M, TI, NT>), dim3((unsigned)blocks), dim3(NT), smem, c->stream, rhs, spec,
                      reinterpret_cast<const double2*>(c->xtw), nlines);
   X3D_CHECK_HIP(hipGetLastError());
