@@ -103,7 +103,7 @@ struct x3d_ctx {
   int ncu = 0;                      // compute units of the context's device
   int xcd_remap = 1;                // line blocks regrouped per XCD (X3D_XCD_REMAP=0 turns it off)
   int spl_wg[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident work-groups/CU of the Poisson line kernels [FP64/FP32][mode]
-  int rfx_wg[2] = {0, 0};                        // and of the x r2c kernel [FP64/FP32]
+  int rfx_wg[2][2] = {{0, 0}, {0, 0}};           // and of the x r2c / c2r kernels [FP64/FP32][forward/inverse]
   std::vector<double> ax, bx, ay, by, az, bz;
   std::vector<double> xk2, yk2, zk2, exs, eys, ezs;
   // step scratch fields (allocated on first use)
