@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -45,6 +46,38 @@ int set_streams(x3d_batch* b, hipStream_t s)
   return X3D_OK;
 }
 
+// Poisson of every grid.  One grid's solve is a few small kernels that leave
+// most compute units idle, so the grids go round-robin onto the side streams,
+// forked from the batch stream and joined back to it: kernels of different
+// grids run side by side.  Each context runs on its side stream for the
+// call and is given the batch stream back afterwards.
+int batch_poisson(x3d_batch* b)
+{
+  const int ns = (int)b->pstreams.size();
+  if (ns > 0) {
+    X3D_CHECK_HIP(hipEventRecord(b->fork_ev, b->stream));
+    for (hipStream_t s : b->pstreams) X3D_CHECK_HIP(hipStreamWaitEvent(s, b->fork_ev, 0));
+  }
+  int rc = X3D_OK;
+  for (int i = 0; !rc && i < b->ng; ++i) {
+    x3d_ctx* c = b->g[i];
+    if (ns > 0) rc = x3d_set_stream(c, b->pstreams[i % ns]);
+    void* pp = b->field(6, i);
+    if (!rc) rc = b->prec == 8 ? x3d_poisson(c, static_cast<double*>(pp)) : x3d_poisson_f32(c, static_cast<float*>(pp));
+  }
+  if (ns > 0) {
+    // join every side stream, also after an error, so that no capture or
+    // later batch work is left unordered against them
+    for (int k = 0; k < ns; ++k) {
+      if (hipEventRecord(b->join_ev[k], b->pstreams[k]) != hipSuccess && !rc) rc = X3D_ERR_HIP;
+      if (hipStreamWaitEvent(b->stream, b->join_ev[k], 0) != hipSuccess && !rc) rc = X3D_ERR_HIP;
+    }
+    const int rs = set_streams(b, b->stream);
+    if (!rc) rc = rs;
+  }
+  return rc;
+}
+
 // One step of every grid: transeq (Euler folded in), divergence, Poisson per
 // grid, gradp with cor_vel.
 int batch_step_body(x3d_batch* b, double gdt)
@@ -56,10 +89,7 @@ int batch_step_body(x3d_batch* b, double gdt)
   auto passes = [b](int which) { return b->prec == 8 ? batch_passes<double>(b, which) : batch_passes<float>(b, which); };
   int rc = passes(0);
   if (!rc) rc = passes(1);
-  for (int i = 0; !rc && i < b->ng; ++i) {
-    void* pp = b->field(6, i);
-    rc = b->prec == 8 ? x3d_poisson(b->g[i], static_cast<double*>(pp)) : x3d_poisson_f32(b->g[i], static_cast<float*>(pp));
-  }
+  if (!rc) rc = batch_poisson(b);
   if (!rc) rc = passes(2);
   return rc;
 }
@@ -158,6 +188,30 @@ int x3d_batch_create(int ngrids, const int* dims, const double* lengths, int pre
     x3d_batch_destroy(b);
     return X3D_ERR_ALLOC;
   }
+  // side streams of the Poisson solves (default priority; X3D_BATCH_PSTREAMS
+  // sets their number, 0 runs the solves in order on the batch stream)
+  int nps = std::min(4, ngrids);
+  if (const char* ps = std::getenv("X3D_BATCH_PSTREAMS")) nps = std::max(0, std::min(16, std::atoi(ps)));
+  if (nps > 0) {
+    if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
+      x3d_batch_destroy(b);
+      return X3D_ERR_HIP;
+    }
+    for (int k = 0; k < nps; ++k) {
+      hipStream_t s = nullptr;
+      hipEvent_t e = nullptr;
+      if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        x3d_batch_destroy(b);
+        return X3D_ERR_HIP;
+      }
+      b->pstreams.push_back(s);
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        x3d_batch_destroy(b);
+        return X3D_ERR_HIP;
+      }
+      b->join_ev.push_back(e);
+    }
+  }
   (void)hipMemset(b->store, 0, 10 * b->total * (size_t)precision);
   *out = b;
   return X3D_OK;
@@ -167,6 +221,12 @@ int x3d_batch_destroy(x3d_batch* b)
 {
   if (!b) return X3D_ERR_ARG;
   graph_release(b);
+  for (hipStream_t s : b->pstreams) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
+  for (hipEvent_t e : b->join_ev) (void)hipEventDestroy(e);
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   for (x3d_ctx* c : b->g) x3d_destroy(c);
   if (b->store) (void)hipFree(b->store);
   if (b->dargs) (void)hipFree(b->dargs);

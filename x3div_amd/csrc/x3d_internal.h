@@ -164,6 +164,11 @@ struct x3d_batch {
   double built_gdt = 0.0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // side streams of the per-grid Poisson solves (forked from and joined back
+  // to the batch stream each step), and their fork / join events
+  std::vector<hipStream_t> pstreams;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;
   void* field(int f, int i) const { return (char*)store + ((size_t)f * total + off[i]) * (size_t)prec; }
 };
 
