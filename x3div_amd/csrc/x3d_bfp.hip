@@ -8,8 +8,10 @@
 // stay near 1e-7 of FP64 while every pass moves as many bytes as FP32.
 //
 // Entry points: conversion from and to FP64, every operator, and the step
-// (the fused passes on BFP32 fields; the Poisson solve runs in FP64 on the
-// decoded right-hand side and its result is stored back as BFP32).
+// (the fused passes on BFP32 fields; the Poisson solve runs in FP64, its x
+// line kernels reading and writing the BFP32 words, or on a decoded copy of
+// the right-hand side whose result is encoded back where those kernels do
+// not apply).
 #include "x3d_line.h"
 
 namespace x3d {
@@ -100,12 +102,17 @@ int x3d_step_bfp(x3d_ctx* c, double gdt, int32_t* ux, int32_t* uy, int32_t* uz, 
   auto B = [](int32_t* p) { return reinterpret_cast<bfp32*>(p); };
   X3D_TRY(fused_momentum<bfp32>(c, gdt, 1, B(ux), B(uy), B(uz), B(dux), B(duy), B(duz)));
   X3D_TRY(fused_divergence<bfp32>(c, B(pp3), B(ux), B(uy), B(uz)));
-  // the work arrays are free between the divergence and the gradient passes
-  X3D_TRY(ensure_work(c));
-  double* rhs = c->work[0];
-  X3D_TRY(bfp_decode(c, B(pp3), rhs));
-  X3D_TRY(poisson_solve(c, rhs));
-  X3D_TRY(bfp_encode(c, rhs, B(pp3)));
+  if (c->xsplit) {
+    // the x line kernels of Poisson decode and encode the words themselves
+    X3D_TRY(poisson_solve_bfp(c, pp3));
+  } else {
+    // the work arrays are free between the divergence and the gradient passes
+    X3D_TRY(ensure_work(c));
+    double* rhs = c->work[0];
+    X3D_TRY(bfp_decode(c, B(pp3), rhs));
+    X3D_TRY(poisson_solve(c, rhs));
+    X3D_TRY(bfp_encode(c, rhs, B(pp3)));
+  }
   X3D_TRY(mark_pass(c, 7));
   X3D_TRY(fused_gradp<bfp32>(c, B(px), B(py), B(pz), B(pp3), B(ux), B(uy), B(uz), 1));
   return X3D_OK;

@@ -103,7 +103,7 @@ struct x3d_ctx {
   int ncu = 0;                      // compute units of the context's device
   int xcd_remap = 1;                // line blocks regrouped per XCD (X3D_XCD_REMAP=0 turns it off)
   int spl_wg[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident work-groups/CU of the Poisson line kernels [FP64/FP32][mode]
-  int rfx_wg[2][2] = {{0, 0}, {0, 0}};           // and of the x r2c / c2r kernels [FP64/FP32][forward/inverse]
+  int rfx_wg[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // and of the x r2c / c2r kernels [FP64/FP32/BFP32][forward/inverse]
   std::vector<double> ax, bx, ay, by, az, bz;
   std::vector<double> xk2, yk2, zk2, exs, eys, ezs;
   // step scratch fields (allocated on first use)
@@ -248,6 +248,10 @@ int line_plan_fused32(int n, bool line_fast, LinePlan* out);
 int poisson_setup(x3d_ctx* c);
 void poisson_release(x3d_ctx* c);
 int poisson_solve(x3d_ctx* c, double* rhs);
+// BFP32 right-hand side, solved in FP64 and stored back as BFP32 words, the
+// decode in the x r2c and the encode in the x c2r (contexts whose Poisson
+// runs the line kernels in x, y and z; else X3D_ERR_UNSUPPORTED)
+int poisson_solve_bfp(x3d_ctx* c, int32_t* rhs);
 
 int ensure_work(x3d_ctx* c);
 

@@ -531,6 +531,42 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
   }
 }
 
+// BFP32 fields (the word format of the BFP32 entry points: a 16-point
+// x-segment shares exponent e, word = (q << 1) | b with q = rint(v 2^(30-e))
+// and b = bit k of e + 128 in the segment's points k < 8) read and written two
+// neighbouring points per lane: a segment is then 8 lanes, and its exponent
+// bits sit in the first four lanes' two words each.
+__device__ __forceinline__ unsigned bfp_spread4(unsigned x)
+{
+  x = (x | (x << 2)) & 0x33u;
+  return (x | (x << 1)) & 0x55u;
+}
+// Values of this lane's two words (every lane of the segment takes part).
+__device__ __forceinline__ double2 bfp_pair_value(int2 w)
+{
+  const unsigned long long m0 = __ballot(w.x & 1), m1 = __ballot(w.y & 1);
+  const int g = (int)(threadIdx.x & 63) & ~7;
+  const unsigned a = (unsigned)(m0 >> g) & 0xFu, b = (unsigned)(m1 >> g) & 0xFu;  // points 0,2,4,6 / 1,3,5,7
+  const int eb = (int)(bfp_spread4(a) | (bfp_spread4(b) << 1));
+  return double2{ldexp((double)(w.x >> 1), eb - 158), ldexp((double)(w.y >> 1), eb - 158)};
+}
+// Words of this lane's two values: the segment's exponent from the largest
+// FP64 exponent field over its 8 lanes, as the single-point encoder forms it.
+__device__ __forceinline__ int2 bfp_pair_word(double2 v)
+{
+  unsigned h = max((unsigned)__double2hiint(v.x) & 0x7FFFFFFFu, (unsigned)__double2hiint(v.y) & 0x7FFFFFFFu);
+  h = max(h, (unsigned)__shfl_xor((int)h, 1));
+  h = max(h, (unsigned)__shfl_xor((int)h, 2));
+  h = max(h, (unsigned)__shfl_xor((int)h, 4));
+  int e = (int)(h >> 20) - 1022;
+  e = min(max(e, -126), 127);
+  int q0 = (int)rint(ldexp(v.x, 30 - e)), q1 = (int)rint(ldexp(v.y, 30 - e));
+  q0 = min(max(q0, -1073741824), 1073741823);
+  q1 = min(max(q1, -1073741824), 1073741823);
+  const unsigned eb = (unsigned)(e + 128), k = 2u * (threadIdx.x & 7u);  // points k, k + 1
+  return int2{(int)(((unsigned)q0 << 1) | ((eb >> k) & 1u)), (int)(((unsigned)q1 << 1) | ((eb >> (k + 1)) & 1u))};
+}
+
 // Forward x transform (r2c) of every x-line: a real line of nx = 2M points
 // is read as M complex points z_j = x_{2j} + i x_{2j+1} (its own memory,
 // one 2-word load per point), one length-M DFT gives Z, and the half
@@ -541,8 +577,8 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
 // +gridDim.x, ...  Loads and stores run along the lines (coalesced), and the
 // column layout in LDS, buf[point * ST + line], has the odd row stride
 // ST = TI + 1 so that those accesses spread over the banks.
-template <typename C, int M, int TI, int NT>
-__global__ __launch_bounds__(NT) void k_rfft_x(const typename CplxT<C>::R* __restrict__ rhs, C* __restrict__ spec,
+template <typename C, int M, int TI, int NT, bool BFP = false>
+__global__ __launch_bounds__(NT) void k_rfft_x(const void* __restrict__ rhs, C* __restrict__ spec,
                                                const double2* __restrict__ tw, long long nlines)
 {
 #pragma clang fp contract(off)
@@ -558,12 +594,15 @@ __global__ __launch_bounds__(NT) void k_rfft_x(const typename CplxT<C>::R* __res
   const int tid = threadIdx.x;
   for (int e = tid; e < NTW; e += NT) twl[e] = tw[2 * e];
   for (int e = tid; e < M; e += NT) wn[e] = tw[e];
-  const C* in = reinterpret_cast<const C*>(rhs);
+  // (BFP32: the words of two points per lane; decoded when stored to LDS,
+  // where the loads are waited on anyway)
+  using PT = typename std::conditional<BFP, int2, C>::type;
+  const PT* in = reinterpret_cast<const PT*>(rhs);
   const long long ntiles = (nlines + TI - 1) / TI;
   // the next tile's points are loaded into registers while this one is
   // transformed (the transforms read LDS only, so they do not wait on them)
   constexpr int E = (NE + NT - 1) / NT;
-  C P[E];
+  PT P[E];
   auto fetch = [&](long long tile) {
     const long long l0 = tile * TI;
 #pragma unroll
@@ -581,7 +620,10 @@ __global__ __launch_bounds__(NT) void k_rfft_x(const typename CplxT<C>::R* __res
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q, lt = e / M, j = e - lt * M;
-      if (e < NE) buf[j * ST + lt] = P[q];
+      C v;
+      if constexpr (BFP) v = bfp_pair_value(P[q]);  // every lane: NE is a multiple of 64
+      else v = P[q];
+      if (e < NE) buf[j * ST + lt] = v;
     }
     // (the first barrier also orders the twiddle copy before its readers)
     __syncthreads();
@@ -615,8 +657,8 @@ __global__ __launch_bounds__(NT) void k_rfft_x(const typename CplxT<C>::R* __res
 // X_k, X_{M-k} (k <= M/2: lanes read one run upwards and one downwards, both
 // coalesced) and folds it in registers into Z_k and Z_{M-k}.  Tiles, LDS
 // layout, persistence and prefetch as k_rfft_x.
-template <typename C, int M, int TI, int NT>
-__global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, typename CplxT<C>::R* __restrict__ rhs,
+template <typename C, int M, int TI, int NT, bool BFP = false>
+__global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, void* __restrict__ rhs,
                                                 const double2* __restrict__ tw, long long nlines)
 {
 #pragma clang fp contract(off)
@@ -636,7 +678,8 @@ __global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, type
   // the fold reads W^k before the tile loop's first barrier: every
   // wavefront waits here for the threads that wrote the tables
   __syncthreads();
-  C* out = reinterpret_cast<C*>(rhs);
+  using OT = typename std::conditional<BFP, int2, C>::type;  // BFP32: two words per lane
+  OT* out = reinterpret_cast<OT*>(rhs);
   const long long ntiles = (nlines + TI - 1) / TI;
   constexpr int E = (NEP + NT - 1) / NT;
   C PA[E], PB[E];  // X_k, X_{M-k}
@@ -693,9 +736,13 @@ __global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, type
     __syncthreads();
     if (tile + (long long)gridDim.x < ntiles) fetch(tile + gridDim.x);
     line_dft<C, M, TI, NT, ST>(buf, twl, true);
+    // (BFP32: every lane of a segment encodes; M TI is a multiple of 64)
     for (int e = tid; e < M * TI; e += NT) {
       const int lt = e / M, j = e - lt * M;
-      if (l0 + lt < nlines) out[(l0 + lt) * M + j] = buf[j * ST + lt];
+      OT w;
+      if constexpr (BFP) w = bfp_pair_word(buf[j * ST + lt]);
+      else w = buf[j * ST + lt];
+      if (l0 + lt < nlines) out[(l0 + lt) * M + j] = w;
     }
     // buf is refilled by the next tile
     __syncthreads();
@@ -850,14 +897,15 @@ constexpr size_t rfx_lds_bytes()
   return (size_t)M * (RfxTile<C, M>::TI + 1) * sizeof(C) + (size_t)((M / 2 > 0 ? M / 2 : 1) + M) * sizeof(double2);
 }
 
-template <typename C, int M, bool INV>
-static int rfx_run(x3d_ctx* c, typename CplxT<C>::R* rhs, C* spec, bool prepare)
+template <typename C, int M, bool INV, bool BFP>
+static int rfx_run(x3d_ctx* c, void* rhs, C* spec, bool prepare)
 {
   constexpr int TI = RfxTile<C, M>::TI, NT = RfxTile<C, M>::NT;
   constexpr size_t smem = rfx_lds_bytes<C, M, INV>();
   static_assert(smem <= 160 * 1024, "x r2c / c2r tile fits the LDS");
-  const void* fn = INV ? (const void*)k_irfft_x<C, M, TI, NT> : (const void*)k_rfft_x<C, M, TI, NT>;
-  int& per_cu = c->rfx_wg[sizeof(C) == 16 ? 0 : 1][INV ? 1 : 0];
+  static_assert(!BFP || (sizeof(C) == 16 && M % 8 == 0), "BFP32: FP64 arithmetic, whole 16-point segments");
+  const void* fn = INV ? (const void*)k_irfft_x<C, M, TI, NT, BFP> : (const void*)k_rfft_x<C, M, TI, NT, BFP>;
+  int& per_cu = c->rfx_wg[BFP ? 2 : (sizeof(C) == 16 ? 0 : 1)][INV ? 1 : 0];
   if (prepare) {
     if (smem > 65536) X3D_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     int nb = 0;
@@ -872,28 +920,32 @@ static int rfx_run(x3d_ctx* c, typename CplxT<C>::R* rhs, C* spec, bool prepare)
   const long long cap = (long long)(c->ncu > 0 ? c->ncu : 1) * per_cu;
   const long long blocks = ntiles < cap ? ntiles : cap;
   if constexpr (INV)
-    hipLaunchKernelGGL((k_irfft_x<C, M, TI, NT>), dim3((unsigned)blocks), dim3(NT), smem, c->stream, spec, rhs,
+    hipLaunchKernelGGL((k_irfft_x<C, M, TI, NT, BFP>), dim3((unsigned)blocks), dim3(NT), smem, c->stream, spec, rhs,
                        reinterpret_cast<const double2*>(c->xtw), nlines);
   else
-    hipLaunchKernelGGL((k_rfft_x<C, M, TI, NT>), dim3((unsigned)blocks), dim3(NT), smem, c->stream, rhs, spec,
-                       reinterpret_cast<const double2*>(c->xtw), nlines);
+    hipLaunchKernelGGL((k_rfft_x<C, M, TI, NT, BFP>), dim3((unsigned)blocks), dim3(NT), smem, c->stream,
+                       (const void*)rhs, spec, reinterpret_cast<const double2*>(c->xtw), nlines);
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
 
 // x r2c (INV false: rhs -> spec) or c2r (true: spec -> rhs), nx = 2M with M
 // one of the line kernels' lengths up to 512
-template <typename C, bool INV>
-static int launch_rfx(x3d_ctx* c, typename CplxT<C>::R* rhs, C* spec, bool prepare = false)
+template <typename C, bool INV, bool BFP = false>
+static int launch_rfx(x3d_ctx* c, void* rhs, C* spec, bool prepare = false)
 {
   switch (c->nx / 2) {
-#define X3D_RFX(MM) case MM: return rfx_run<C, MM, INV>(c, rhs, spec, prepare);
+#define X3D_RFX(MM)                                                                   \
+  case MM:                                                                            \
+    if constexpr (!BFP || MM % 8 == 0) return rfx_run<C, MM, INV, BFP>(c, rhs, spec, prepare); \
+    break;
     X3D_RFX(8) X3D_RFX(16) X3D_RFX(32) X3D_RFX(64) X3D_RFX(128) X3D_RFX(256) X3D_RFX(512)
     X3D_RFX(24) X3D_RFX(40) X3D_RFX(48) X3D_RFX(80) X3D_RFX(96) X3D_RFX(160) X3D_RFX(192) X3D_RFX(320)
     X3D_RFX(384)
 #undef X3D_RFX
     default: return X3D_ERR_UNSUPPORTED;
   }
+  return X3D_ERR_UNSUPPORTED;
 }
 
 template <typename C>
@@ -1045,6 +1097,10 @@ int poisson_setup(x3d_ctx* c)
   if (c->xsplit) {
     X3D_TRY_RC((spl_prepare<double2, 1>(c, c->ny)));
     X3D_TRY_RC((launch_rfx<double2, false>(c, nullptr, nullptr, true)));
+    if (c->nx % 16 == 0) {  // the BFP32 form's x kernels
+      X3D_TRY_RC((launch_rfx<double2, false, true>(c, nullptr, nullptr, true)));
+      X3D_TRY_RC((launch_rfx<double2, true, true>(c, nullptr, nullptr, true)));
+    }
   }
   c->fft_ready = true;
   if (hipfftSetStream(c->fft_r2c, c->stream) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
@@ -1186,6 +1242,16 @@ int poisson_solve(x3d_ctx* c, double* rhs)
   X3D_CHECK_HIP(hipGetLastError());
   if (hipfftExecZ2D(c->fft_c2r, (hipfftDoubleComplex*)c->spec, rhs) != HIPFFT_SUCCESS) return X3D_ERR_FFT;
   return X3D_OK;
+}
+
+int poisson_solve_bfp(x3d_ctx* c, int32_t* rhs)
+{
+  if (!c->fft_ready || !c->xsplit || c->nx % 16) return X3D_ERR_UNSUPPORTED;
+  X3D_TRY_RC((launch_rfx<double2, false, true>(c, rhs, c->spec)));
+  X3D_TRY_RC((launch_poisson_y<double2, 1>(c, c->spec)));
+  X3D_TRY_RC(launch_poisson_z<double2>(c, c->spec));
+  X3D_TRY_RC((launch_poisson_y<double2, 2>(c, c->spec)));
+  return launch_rfx<double2, true, true>(c, rhs, c->spec);
 }
 
 }  // namespace x3d
