@@ -72,3 +72,46 @@ def test_hand_worked_words():
     u = np.full(16, np.nextafter(2.0, 0.0))
     w = bm.encode(u).astype(np.int64)
     assert bm.exponents(w)[0] == 1 and (w[0] >> 1) == 2 ** 30 - 1
+
+
+def _pair_encode(u):
+    """The x-line Poisson kernels' encoder, lane by lane: lane i of a
+    segment's 8 holds points 2i, 2i+1; the exponent is taken from the largest
+    FP64 exponent field of the 16 (sign bit cleared), and point k's word
+    carries bit k of e + 128."""
+    hi = (u.view(np.uint64) >> np.uint64(32)).astype(np.int64) & 0x7FFFFFFF
+    w = np.zeros(u.size, dtype=np.int64)
+    for s0 in range(0, u.size, 16):
+        e = min(max(int(hi[s0:s0 + 16].max() >> 20) - 1022, -126), 127)
+        for i in range(8):
+            for t in range(2):
+                k = 2 * i + t
+                q = int(np.rint(np.ldexp(u[s0 + k], 30 - e)))
+                q = min(max(q, -(2 ** 30)), 2 ** 30 - 1)
+                w[s0 + k] = ((q << 1) | (((e + 128) >> k) & 1)) & 0xFFFFFFFF
+    return w.astype(np.uint32).view(np.int32)
+
+
+def _pair_decode(w):
+    """Their decoder: the bits of points 0, 2, 4, 6 (the first words of lanes
+    0-3) and of points 1, 3, 5, 7 (their second words), interleaved."""
+    def spread(x):
+        x = (x | (x << 2)) & 0x33
+        return (x | (x << 1)) & 0x55
+    out = np.zeros(w.size)
+    for s0 in range(0, w.size, 16):
+        seg = w[s0:s0 + 16].astype(np.int64)
+        a = sum((int(seg[2 * i]) & 1) << i for i in range(4))
+        b = sum((int(seg[2 * i + 1]) & 1) << i for i in range(4))
+        eb = spread(a) | (spread(b) << 1)
+        out[s0:s0 + 16] = np.ldexp((seg >> 1).astype(np.float64), eb - 158)
+    return out
+
+
+def test_two_points_per_lane_coder_equals_the_format():
+    rng = np.random.default_rng(11)
+    for u in _fields(rng):
+        flat = np.ascontiguousarray(u).reshape(-1)
+        w = bm.encode(u).reshape(-1)
+        assert np.array_equal(_pair_encode(flat), w)
+        assert np.array_equal(_pair_decode(w), bm.decode(w.reshape(u.shape)).reshape(-1))
