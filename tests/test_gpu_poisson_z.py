@@ -26,10 +26,15 @@ NZ = [8, 16, 32, 64, 128, 256, 512, 1024, 24, 40, 48, 96, 160, 384, 768]
 
 
 @pytest.mark.parametrize("nz", NZ)
-def test_poisson_fused_z(torch_cuda, nz):
+@pytest.mark.parametrize("xline", [False, True], ids=["x-hipfft", "x-line"])
+def test_poisson_fused_z(torch_cuda, nz, xline):
+    """xline: an x length the x r2c line kernel takes (then the forward y
+    stage is a line kernel too); else hipFFT's 2-D forward plan."""
     import torch
     from x3div_amd import X3D
     nx, ny = (24, 16) if nz <= 256 else (12, 8)
+    if xline:
+        nx = 32 if nz <= 256 else 16
     print(f"\n[poisson z {nx}x{ny}x{nz}]", flush=True)
     ctx = X3D(nx, ny, nz, 1.0, 2.0, 3.0)
     orc = Oracle(nx, ny, nz, 1.0, 2.0, 3.0)
@@ -52,12 +57,17 @@ def test_poisson_fused_z(torch_cuda, nz):
 
 
 @pytest.mark.parametrize("ny", [8, 32, 128, 512, 1024, 24, 48, 80, 96, 160, 192, 320])
-def test_poisson_fused_z_y_lengths(torch_cuda, ny):
-    """The fused z stage under 2-D plans of several y lengths (the 2-D
-    hipFFT layout of the half spectrum that the z kernel reads)."""
+@pytest.mark.parametrize("xline", [False, True], ids=["x-hipfft", "x-line"])
+def test_poisson_fused_z_y_lengths(torch_cuda, ny, xline):
+    """The fused z stage and the y stages for several y lengths: the forward
+    y stage as hipFFT's 2-D plan (with its x r2c) or as the y-line kernel
+    after the x r2c line kernel (xline), the inverse y stage as the y-line
+    kernel where the length allows."""
     import torch
     from x3div_amd import X3D
     nx, nz = (20, 16) if ny <= 128 else (10, 8)
+    if xline:
+        nx = 32 if ny <= 128 else 16
     print(f"\n[poisson y {nx}x{ny}x{nz}]", flush=True)
     ctx = X3D(nx, ny, nz, 2.0, 1.0, 3.0)
     orc = Oracle(nx, ny, nz, 2.0, 1.0, 3.0)
