@@ -56,7 +56,7 @@ def test_poisson_fused_z(torch_cuda, nz, xline):
     assert e32 < 2e-6, e32
 
 
-@pytest.mark.parametrize("ny", [8, 32, 128, 512, 1024, 24, 48, 80, 96, 160, 192, 320])
+@pytest.mark.parametrize("ny", [8, 32, 128, 512, 1024, 24, 48, 80, 96, 160, 192, 320, 768])
 @pytest.mark.parametrize("xline", [False, True], ids=["x-hipfft", "x-line"])
 def test_poisson_fused_z_y_lengths(torch_cuda, ny, xline):
     """The fused z stage and the y stages for several y lengths: the forward

@@ -326,6 +326,37 @@ __global__ __launch_bounds__(NT) void k_poisson_z(C* __restrict__ spec, const do
   }
 }
 
+// y stages for lines longer than the pipelined kernels take (768 and 1024
+// points): one tile of TI x-columns of one z-plane per work-group, loaded,
+// transformed forward (MODE 1) or inverse (MODE 2) in LDS, stored.
+template <typename C, int N, int TI, int NT, int MODE>
+__global__ __launch_bounds__(NT) void k_y_plain(C* __restrict__ spec, const double2* __restrict__ tw, int nx, int ny)
+{
+#pragma clang fp contract(off)
+  extern __shared__ unsigned char smem_raw[];
+  C* buf = reinterpret_cast<C*>(smem_raw);
+  const int nxh = nx / 2 + 1;
+  const int tiles = (nxh + TI - 1) / TI;
+  const int o = blockIdx.x / tiles, i0 = (blockIdx.x - o * tiles) * TI;
+  const long long S = nxh;  // between y points
+  const long long base = (long long)o * nxh * ny + i0;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < N * TI; e += NT) {
+    const int k = e / TI, col = e % TI;
+    C v;
+    v.x = 0;
+    v.y = 0;
+    if (i0 + col < nxh) v = spec[k * S + base + col];
+    buf[e] = v;
+  }
+  __syncthreads();
+  line_dft<C, N, TI, NT>(buf, tw, MODE == 2);
+  for (int e = tid; e < N * TI; e += NT) {
+    const int k = e / TI, col = e % TI;
+    if (i0 + col < nxh) spec[k * S + base + col] = buf[e];
+  }
+}
+
 // Persistent, software-pipelined line kernels over the half spectrum.  With
 // one work-group per compute unit (a 128 KiB tile) the kernel above loads,
 // then transforms, then stores, and the memory system idles during the
@@ -856,12 +887,31 @@ static int launch_pz_n(x3d_ctx* c, C* spec)
   return X3D_OK;
 }
 
+template <typename C, int N, int MODE>
+static int launch_y_plain(x3d_ctx* c, C* spec)
+{
+  constexpr int TI = LineTile<C, N>::TI, NT = LineTile<C, N>::NT;
+  constexpr size_t smem = (size_t)N * TI * sizeof(C);
+  static_assert(smem <= 160 * 1024, "y tile fits the LDS");
+  const int nxh = c->nx / 2 + 1;
+  const unsigned blocks = (unsigned)(((nxh + TI - 1) / TI) * c->nz);
+  if (smem > 65536)
+    (void)hipFuncSetAttribute((const void*)k_y_plain<C, N, TI, NT, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)smem);
+  hipLaunchKernelGGL((k_y_plain<C, N, TI, NT, MODE>), dim3(blocks), dim3(NT), smem, c->stream, spec,
+                     reinterpret_cast<const double2*>(c->ytw), c->nx, c->ny);
+  X3D_CHECK_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
 // y stages of the 3-D transform (MODE 1 forward, 2 inverse), for the y
-// lengths ysplit_ok admits
+// lengths ysplit_ok admits (768 and 1024 points: the one-tile kernel)
 template <typename C, int MODE>
 static int launch_poisson_y(x3d_ctx* c, C* spec)
 {
   switch (c->ny) {
+    case 768: return launch_y_plain<C, 768, MODE>(c, spec);
+    case 1024: return launch_y_plain<C, 1024, MODE>(c, spec);
     case 8: return launch_spl<C, 8, MODE>(c, spec, c->ytw);
     case 16: return launch_spl<C, 16, MODE>(c, spec, c->ytw);
     case 32: return launch_spl<C, 32, MODE>(c, spec, c->ytw);
@@ -988,7 +1038,7 @@ static bool smooth_len(int n)
   return false;
 }
 static bool zsplit_ok(int nz) { return nz >= 8 && nz <= 1024 && smooth_len(nz); }
-static bool ysplit_ok(int ny) { return ny >= 8 && ny <= 512 && smooth_len(ny); }
+static bool ysplit_ok(int ny) { return ny >= 8 && ny <= 1024 && smooth_len(ny) && ny != 640; }
 static bool xsplit_ok(int nx) { return nx % 2 == 0 && nx / 2 >= 8 && nx / 2 <= 512 && smooth_len(nx / 2); }
 
 // Table of e^{-2 pi i t / n}, t < n, on the device.
