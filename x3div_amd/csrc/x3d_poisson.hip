@@ -799,12 +799,12 @@ static bool spl_off32_ok(const x3d_ctx* c)
   return (long long)(LineTile<C, N>::NT / LineTile<C, N>::TI) * S * (long long)sizeof(C) < (1LL << 32);
 }
 
-// The pipelined kernels cover lines of up to 512 points (longer ones leave
-// too few registers for the prefetch) whose LDS fits a compute unit.
+// The pipelined kernels cover lines of up to 768 points whose LDS fits a
+// compute unit (at 1024 points the prefetch spills).
 template <typename C, int N, int MODE>
 constexpr bool spl_ok()
 {
-  return N >= 8 && N <= 512 && spl_lds_bytes<C, N, LineTile<C, N>::TI, MODE>() <= 160 * 1024;
+  return N >= 8 && N <= 768 && spl_lds_bytes<C, N, LineTile<C, N>::TI, MODE>() <= 160 * 1024;
 }
 
 // Slot of a line kernel's resident work-groups per compute unit in the
@@ -839,7 +839,7 @@ static int spl_prepare(x3d_ctx* c, int n)
     case NN: if constexpr (spl_ok<C, NN, MODE>()) return spl_prepare_n<C, NN, MODE>(c); break;
     X3D_SPL_N(8) X3D_SPL_N(16) X3D_SPL_N(32) X3D_SPL_N(64) X3D_SPL_N(128) X3D_SPL_N(256) X3D_SPL_N(512)
     X3D_SPL_N(24) X3D_SPL_N(40) X3D_SPL_N(48) X3D_SPL_N(80) X3D_SPL_N(96) X3D_SPL_N(160) X3D_SPL_N(192)
-    X3D_SPL_N(320) X3D_SPL_N(384)
+    X3D_SPL_N(320) X3D_SPL_N(384) X3D_SPL_N(640) X3D_SPL_N(768) X3D_SPL_N(1024)
 #undef X3D_SPL_N
     default: break;
   }
@@ -910,8 +910,12 @@ template <typename C, int MODE>
 static int launch_poisson_y(x3d_ctx* c, C* spec)
 {
   switch (c->ny) {
-    case 768: return launch_y_plain<C, 768, MODE>(c, spec);
-    case 1024: return launch_y_plain<C, 1024, MODE>(c, spec);
+    case 768:
+      if constexpr (spl_ok<C, 768, MODE>()) return launch_spl<C, 768, MODE>(c, spec, c->ytw);
+      return launch_y_plain<C, 768, MODE>(c, spec);
+    case 1024:
+      if constexpr (spl_ok<C, 1024, MODE>()) return launch_spl<C, 1024, MODE>(c, spec, c->ytw);
+      return launch_y_plain<C, 1024, MODE>(c, spec);
     case 8: return launch_spl<C, 8, MODE>(c, spec, c->ytw);
     case 16: return launch_spl<C, 16, MODE>(c, spec, c->ytw);
     case 32: return launch_spl<C, 32, MODE>(c, spec, c->ytw);
