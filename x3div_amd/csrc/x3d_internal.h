@@ -94,9 +94,9 @@ struct x3d_ctx {
   bool fft32_ready = false;
   double2* spec = nullptr;          // (nx/2+1) * ny * nz complex
   double* ktab = nullptr;           // device 1-D tables, see poisson source
-  bool zsplit = false;              // 2-D hipFFT plans + fused z stage (nz a power of two)
+  bool zsplit = false;              // fused z-stage kernel (nz a line-kernel length up to 1024)
   void* ztw = nullptr;              // e^{-2 pi i t / nz}, t < nz (double2)
-  bool ysplit = false;              // and x-only hipFFT plans + y stage kernels (ny a power of two <= 512)
+  bool ysplit = false;              // and y-stage kernels (ny a line-kernel length up to 1024)
   void* ytw = nullptr;              // e^{-2 pi i t / ny}, t < ny (double2)
   bool xsplit = false;              // and the forward x r2c and y DFT as line kernels (X3D_XSPLIT=0: hipFFT 2-D)
   void* xtw = nullptr;              // e^{-2 pi i t / nx}, t < nx (double2)

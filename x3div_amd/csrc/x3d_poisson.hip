@@ -8,7 +8,7 @@
 // K < 1e-16), which is what the spectral kernel applies; a parity test
 // against the statement-by-statement host restatement confirms it.  The
 // multiplier depends on |wavenumber| per axis only, so the halved axis is
-// free: here hipFFT halves x (the contiguous axis).
+// free: here x (the contiguous axis) is halved.
 //
 // K = xk2 (Ty Tz)^2 + yk2 (Tx Tz)^2 + zk2 (Tx Ty)^2 is evaluated per mode
 // from 1-D tables in the reference's operation order with contraction off,
@@ -56,17 +56,18 @@ __global__ void k_spectral(double2* __restrict__ spec, const double* __restrict_
   }
 }
 
-// ---- fused z stage ---------------------------------------------------------
-// With nz a power of two the 3-D transform is split: hipFFT runs the 2-D
-// r2c/c2r over (x, y) of every z-plane, and one kernel does, per z-line of the
-// half spectrum, the forward length-nz DFT, the division above and the
-// inverse DFT, so the spectrum crosses HBM once for the three steps.  When ny
-// is also a power of two (up to 512), the inverse y stage is a kernel of the
-// same kind and hipFFT runs only the inverse x transform.  A work-group takes TI
-// neighbouring x-columns (16 where they fit in LDS: 256-byte rows for FP64),
-// holds the TI lines in LDS and runs Stockham radix-8 passes (a final
-// radix-4 or radix-2 pass when log2 n is not a multiple of 3) with twiddles
-// from a table of e^{-2 pi i t / n}.
+// ---- line stages -------------------------------------------------------------
+// For line lengths 2^a 3^b 5^c the 3-D transform is split into line stages.
+// Forward: the x r2c (k_rfft_x) and the y DFT (k_spec_lines mode 1), or
+// hipFFT's 2-D (x, y) plan where those lengths are not built.  Then one
+// kernel does, per z-line of the half spectrum, the forward length-nz DFT,
+// the division above and the inverse DFT, so the spectrum crosses HBM once
+// for the three steps.  Inverse: the y DFT (mode 2), then the x c2r
+// (k_irfft_x in FP32, hipFFT's batched 1-D c2r in FP64), or hipFFT's 2-D
+// plan.  A work-group takes TI neighbouring x-columns (16 where they fit in
+// LDS: 256-byte rows for FP64), holds the TI lines in LDS and runs Stockham
+// radix-8 passes (then a radix-4 or radix-2 pass, then radix-3 and radix-5
+// passes) with twiddles from a table of e^{-2 pi i t / n}.
 
 template <typename C>
 struct CplxT;
