@@ -298,3 +298,28 @@ def test_non_periodic_y(torch_cuda):
                 assert rel(host(t), ref) < RTOL, (bc, name)
         finally:
             ctx.close()
+
+
+def test_batch_refuses_a_stretched_grid(torch_cuda):
+    """The batched passes carry no stretching vectors: a batch whose grid
+    context was given stretching refuses to step (and steps again once the
+    stretching is turned off)."""
+    import ctypes as C
+    import torch
+    from x3div_amd import X3DBatch
+    from x3div_amd._lib import X3DError
+    b = X3DBatch([(16, 16, 16), (32, 16, 8)])
+    try:
+        b.step(0.0)
+        torch.cuda.synchronize()
+        h = b._ctx(1)
+        ppy, ppyi = vectors(16, 5)
+        a, ai = np.ascontiguousarray(ppy), np.ascontiguousarray(ppyi)
+        assert b.lib.x3d_set_stretching(h, 1, a.ctypes.data, ai.ctypes.data) == 0
+        with pytest.raises(X3DError):
+            b.step(0.0)
+        assert b.lib.x3d_set_stretching(h, 0, None, None) == 0
+        b.step(0.0)
+        torch.cuda.synchronize()
+    finally:
+        b.close()
