@@ -365,7 +365,11 @@ int x3d_step_noalloc(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *u
  * ux uy uz dux duy duz pp3 px py pz (field index 0-9), of FP64 (precision
  * 8) or FP32 (precision 4) values.  Each step pass is one persistent launch
  * per group of grids sharing a line tiling; Poisson runs per grid.  A grid's
- * own context (x3d_batch_context) serves the single-grid entry points.
+ * own context (x3d_batch_context) serves the single-grid entry points; while
+ * any grid's context holds y-stretching (x3d_set_stretching) or operator
+ * tables of the caller's own (x3d_operator_set), x3d_batch_step and
+ * x3d_batch_graph_launch return X3D_ERR_UNSUPPORTED (the batched passes run
+ * the circulant solvers of the library's tables, unstretched).
  * dims: nx, ny, nz per grid; lengths: xlx, yly, zlz per grid, or NULL for
  * 2*pi.  Seeds of x3d_batch_init_fields are seed + grid index. */
 typedef struct x3d_batch x3d_batch;

@@ -323,3 +323,42 @@ def test_batch_refuses_a_stretched_grid(torch_cuda):
         torch.cuda.synchronize()
     finally:
         b.close()
+
+
+def test_batch_refuses_caller_tables(torch_cuda):
+    """The batched passes run the circulant solvers of the library's own
+    tables: a batch whose grid context was given tables of the caller's own
+    (x3d_operator_set) refuses to step and to replay its graph, and steps
+    again once the library's tables are set back."""
+    import ctypes as C
+    import torch
+    from x3div_amd import X3DBatch
+    from x3div_amd._lib import X3DError
+    from x3div_amd.context import _op_id
+    b = X3DBatch([(16, 16, 16), (32, 16, 8)])
+    try:
+        b.step(0.0)
+        torch.cuda.synchronize()
+        b.graph_capture(0.0)
+        b.graph_launch()
+        torch.cuda.synchronize()
+        h, op = b._ctx(0), _op_id("dery")
+        hp = C.POINTER(C.c_double)
+        f, s, w, p = (np.zeros(16) for _ in range(4))
+        alfa, n, npaire = C.c_double(), C.c_int(), C.c_int()
+        assert b.lib.x3d_operator_get(h, op, *(a.ctypes.data_as(hp) for a in (f, s, w, p)),
+                                      C.byref(alfa), C.byref(n), C.byref(npaire)) == 0
+        assert n.value == 16
+        f2 = f * (1 + 1e-3)
+        assert b.lib.x3d_operator_set(h, op, *(a.ctypes.data_as(hp) for a in (f2, s, w, p)), alfa.value) == 0
+        with pytest.raises(X3DError):
+            b.step(0.0)
+        with pytest.raises(X3DError):
+            b.graph_launch()
+        assert b.lib.x3d_operator_set(h, op, *(a.ctypes.data_as(hp) for a in (f, s, w, p)), alfa.value) == 0
+        b.step(0.0)
+        b.graph_capture(0.0)
+        b.graph_launch()
+        torch.cuda.synchronize()
+    finally:
+        b.close()

@@ -80,19 +80,21 @@ int batch_poisson(x3d_batch* b)
 
 // One step of every grid: transeq (Euler folded in), divergence, Poisson per
 // grid, gradp with cor_vel.
-// The batched passes carry no y-stretching vectors: a grid whose context was
-// given stretching (x3d_set_stretching through x3d_batch_context) is refused,
-// on every step and graph launch, not only when the launches are built.
-bool any_stretched(const x3d_batch* b)
+// The batched passes run the circulant solvers of the library's own operator
+// tables and carry no y-stretching vectors: a grid whose context was given
+// stretching (x3d_set_stretching) or tables of the caller's own
+// (x3d_operator_set), through x3d_batch_context, is refused on every step and
+// graph launch, not only when the launches are built.
+bool batch_refused(const x3d_batch* b)
 {
   for (const x3d_ctx* c : b->g)
-    if (c->istret) return true;
+    if (c->istret || !fused_ok(c)) return true;
   return false;
 }
 
 int batch_step_body(x3d_batch* b, double gdt)
 {
-  if (any_stretched(b)) return X3D_ERR_UNSUPPORTED;
+  if (batch_refused(b)) return X3D_ERR_UNSUPPORTED;
   if (!b->built || b->built_gdt != gdt) {
     const int rc = b->prec == 8 ? batch_build<double>(b, gdt) : batch_build<float>(b, gdt);
     if (rc) return rc;
@@ -356,7 +358,7 @@ int x3d_batch_graph_capture(x3d_batch* b, double gdt)
 int x3d_batch_graph_launch(x3d_batch* b)
 {
   if (!b) return X3D_ERR_ARG;
-  if (!b->graph_exec || any_stretched(b)) return X3D_ERR_UNSUPPORTED;
+  if (!b->graph_exec || batch_refused(b)) return X3D_ERR_UNSUPPORTED;
   X3D_CHECK_HIP(hipGraphLaunch(b->graph_exec, b->stream));
   return X3D_OK;
 }

@@ -638,9 +638,10 @@ constexpr int kPassDir[9] = {0, 1, 2, 0, 1, 2, 2, 1, 0};
 template <typename T>
 int batch_build(x3d_batch* b, double gdt)
 {
-  // the batched passes carry no stretching vectors
+  // the batched passes carry no stretching vectors and run the circulant
+  // solvers of the library's own tables
   for (const x3d_ctx* g : b->g)
-    if (g->istret) return X3D_ERR_UNSUPPORTED;
+    if (g->istret || !fused_ok(g)) return X3D_ERR_UNSUPPORTED;
   std::vector<unsigned char> host;
   auto put = [&host](const void* p, size_t n) {
     size_t at = (host.size() + 255) & ~size_t(255);
