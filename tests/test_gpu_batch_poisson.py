@@ -1,0 +1,95 @@
+"""Grouped Poisson of the batching form: one launch per stage and line length
+over every grid the line kernels take, the other grids solved one by one.
+
+* Against the per-grid solves (X3D_BATCH_PGROUP=0) on the same fields: in FP32
+  both run the same line kernels with contraction off, so the steps agree bit
+  for bit; in FP64 the per-grid inverse x transform is hipFFT's, the grouped
+  one the line kernel, so they agree to rounding: 1e-13 relative for the
+  velocities and their right-hand sides, 1e-12 for the pressure and its
+  gradient (the oracle bar for those is 1e-11), over one step (a second
+  step's pressure comes from the O(gdt) divergence left after the first
+  projection, about 1e3 times smaller, so the same rounding reads 1e3 times
+  larger relative to it).
+* A batch mixing grouped grids with ones solved alone (an x length the x line
+  kernel is not built for, a 512-point z line) against the oracle at the FP64
+  bar, eager and through a captured graph (replays equal to eager steps bit
+  for bit).
+"""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [(32, 48, 64), (48, 32, 32), (64, 64, 48), (32, 48, 64), (96, 32, 40), (40, 160, 32), (16, 24, 8)]
+MIXED = [(32, 48, 64), (16, 16, 512), (48, 32, 32), (20, 24, 16), (32, 48, 64)]
+SEED = 97
+FIELDS = ("ux", "uy", "uz", "dux", "duy", "duz", "pp3", "px", "py", "pz")
+
+
+def relerr(a, b):
+    a = np.asarray(a, dtype=np.float64); b = np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _batch(dims, prec, grouped, monkeypatch):
+    from x3div_amd import X3DBatch
+    if grouped:
+        monkeypatch.delenv("X3D_BATCH_PGROUP", raising=False)
+    else:
+        monkeypatch.setenv("X3D_BATCH_PGROUP", "0")
+    b = X3DBatch(dims, precision=prec)
+    monkeypatch.delenv("X3D_BATCH_PGROUP", raising=False)
+    b.init_fields(kind=2, seed=SEED)
+    return b
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_grouped_poisson_equals_per_grid(torch_cuda, prec, monkeypatch):
+    print(f"\n[grouped vs per-grid Poisson, {len(DIMS)} grids, precision {prec}]", flush=True)
+    a = _batch(DIMS, prec, True, monkeypatch)
+    b = _batch(DIMS, prec, False, monkeypatch)
+    try:
+        for step in range(2 if prec == 4 else 1):
+            a.step(gdt=1e-3)
+            b.step(gdt=1e-3)
+            for i in range(len(DIMS)):
+                for name in FIELDS:
+                    x, y = a.get(i, name), b.get(i, name)
+                    e = relerr(x, y)
+                    if prec == 4:
+                        assert np.array_equal(x, y), (step, i, DIMS[i], name, e)
+                    else:
+                        tol = 1e-12 if name in ("pp3", "px", "py", "pz") else 1e-13
+                        assert e < tol, (step, i, DIMS[i], name, e)
+    finally:
+        a.close()
+        b.close()
+
+
+def test_mixed_batch_matches_oracle_eager_and_captured(torch_cuda, monkeypatch):
+    print(f"\n[mixed grouped / alone batch {MIXED}]", flush=True)
+    eager = _batch(MIXED, 8, True, monkeypatch)
+    graph = _batch(MIXED, 8, True, monkeypatch)
+    try:
+        eager.step(gdt=1e-3)
+        for i, (nx, ny, nz) in enumerate(MIXED):
+            o = Oracle(nx, ny, nz, 2 * np.pi, 2 * np.pi, 2 * np.pi)
+            u, du, pp, g = o.step(*o.fields(2, SEED + i, 0.0), gdt=1e-3)
+            for name, r in zip(FIELDS[:6], u + du):
+                assert relerr(eager.get(i, name), r) < 1e-12, (i, MIXED[i], name)
+            assert relerr(eager.get(i, "pp3"), pp) < 1e-11, (i, MIXED[i])
+            for name, r in zip(FIELDS[7:], g):
+                assert relerr(eager.get(i, name), r) < 1e-11, (i, MIXED[i], name)
+        graph.graph_capture(gdt=1e-3)
+        for rep in range(3):
+            if rep:
+                eager.step(gdt=1e-3)
+            graph.graph_launch()
+            for i in range(len(MIXED)):
+                for name in FIELDS:
+                    assert np.array_equal(graph.get(i, name), eager.get(i, name)), (rep, i, name)
+    finally:
+        eager.close()
+        graph.close()

@@ -46,22 +46,25 @@ int set_streams(x3d_batch* b, hipStream_t s)
   return X3D_OK;
 }
 
-// Poisson of every grid.  One grid's solve is a few small kernels that leave
-// most compute units idle, so the grids go round-robin onto the side streams,
-// forked from the batch stream and joined back to it: kernels of different
-// grids run side by side.  Each context runs on its side stream for the
-// call and is given the batch stream back afterwards.
+// Poisson of every grid.  The grids the line kernels take run grouped: one
+// persistent launch per stage and line length over all of them, on the batch
+// stream.  The others (psolo) solve one by one; a grid's solve is a few small
+// kernels that leave most compute units idle, so they go round-robin onto the
+// side streams, forked from the batch stream and joined back to it, and run
+// beside each other and the grouped launches.  Each such context runs on its
+// side stream for the call and is given the batch stream back afterwards.
 int batch_poisson(x3d_batch* b)
 {
-  const int ns = (int)b->pstreams.size();
+  const int ns = b->psolo.empty() ? 0 : (int)b->pstreams.size();
   if (ns > 0) {
     X3D_CHECK_HIP(hipEventRecord(b->fork_ev, b->stream));
     for (hipStream_t s : b->pstreams) X3D_CHECK_HIP(hipStreamWaitEvent(s, b->fork_ev, 0));
   }
-  int rc = X3D_OK;
-  for (int i = 0; !rc && i < b->ng; ++i) {
+  int rc = poisson_batch_run(b);
+  for (size_t k = 0; !rc && k < b->psolo.size(); ++k) {
+    const int i = b->psolo[k];
     x3d_ctx* c = b->g[i];
-    if (ns > 0) rc = x3d_set_stream(c, b->pstreams[i % ns]);
+    if (ns > 0) rc = x3d_set_stream(c, b->pstreams[k % ns]);
     void* pp = b->field(6, i);
     if (!rc) rc = b->prec == 8 ? x3d_poisson(c, static_cast<double*>(pp)) : x3d_poisson_f32(c, static_cast<float*>(pp));
   }
@@ -78,8 +81,8 @@ int batch_poisson(x3d_batch* b)
   return rc;
 }
 
-// One step of every grid: transeq (Euler folded in), divergence, Poisson per
-// grid, gradp with cor_vel.
+// One step of every grid: transeq (Euler folded in), divergence, Poisson,
+// gradp with cor_vel.
 // The batched passes run the circulant solvers of the library's own operator
 // tables and carry no y-stretching vectors: a grid whose context was given
 // stretching (x3d_set_stretching) or tables of the caller's own
@@ -201,10 +204,17 @@ int x3d_batch_create(int ngrids, const int* dims, const double* lengths, int pre
     x3d_batch_destroy(b);
     return X3D_ERR_ALLOC;
   }
-  // side streams of the Poisson solves (default priority; X3D_BATCH_PSTREAMS
-  // sets their number, 0 runs the solves in order on the batch stream)
-  int nps = std::min(4, ngrids);
+  // the grouped Poisson launches, and side streams for the grids they do not
+  // take (default priority; X3D_BATCH_PSTREAMS sets their number, 0 runs
+  // those solves in order on the batch stream)
+  if (const int rc = poisson_batch_build(b)) {
+    x3d_batch_destroy(b);
+    return rc;
+  }
+  const int nsolo = (int)b->psolo.size();
+  int nps = std::min(4, nsolo);
   if (const char* ps = std::getenv("X3D_BATCH_PSTREAMS")) nps = std::max(0, std::min(16, std::atoi(ps)));
+  nps = std::min(nps, nsolo);
   if (nps > 0) {
     if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
       x3d_batch_destroy(b);
@@ -243,6 +253,7 @@ int x3d_batch_destroy(x3d_batch* b)
   for (x3d_ctx* c : b->g) x3d_destroy(c);
   if (b->store) (void)hipFree(b->store);
   if (b->dargs) (void)hipFree(b->dargs);
+  if (b->pargs) (void)hipFree(b->pargs);
   delete b;
   return X3D_OK;
 }

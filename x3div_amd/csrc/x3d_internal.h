@@ -164,6 +164,23 @@ struct x3d_batch {
   double built_gdt = 0.0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // Poisson of the grids the line kernels take, grouped: one launch per
+  // stage (0 x r2c, 1 forward y, 2 z, 3 inverse y, 4 x c2r) and line length
+  // over all of them, whose per-grid records sit in pargs; the other grids
+  // (psolo) solve one by one
+  struct PLaunch {
+    const void* fn;
+    unsigned blocks, threads;
+    size_t smem;
+    int stage;
+    const void* tw;    // twiddles of the group's line length
+    size_t grids;      // byte offset of its grid records in pargs
+    size_t bgrid;      // byte offset of its first grid per work-group
+    long long tiles;   // tiles of all its grids
+  };
+  std::vector<PLaunch> plaunch;
+  std::vector<int> psolo;
+  void* pargs = nullptr;
   // side streams of the per-grid Poisson solves (forked from and joined back
   // to the batch stream each step), and their fork / join events
   std::vector<hipStream_t> pstreams;
@@ -252,6 +269,11 @@ int poisson_solve(x3d_ctx* c, double* rhs);
 // decode in the x r2c and the encode in the x c2r (contexts whose Poisson
 // runs the line kernels in x, y and z; else X3D_ERR_UNSUPPORTED)
 int poisson_solve_bfp(x3d_ctx* c, int32_t* rhs);
+// Grouped Poisson of a batch (x3d_poisson.hip): build the launches and their
+// device records (outside any capture; X3D_BATCH_PGROUP=0 solves every grid
+// on its own), and run them on the batch stream.
+int poisson_batch_build(x3d_batch* b);
+int poisson_batch_run(x3d_batch* b);
 
 int ensure_work(x3d_ctx* c);
 

@@ -16,12 +16,29 @@
 // test makes the same decisions.
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <map>
+#include <vector>
 
 #include "x3d_internal.h"
 
 namespace x3d {
 
 static const double PI = 3.14159265358979323846264338327950288;
+
+// One grid of a grouped launch (the batching form: one launch per Poisson
+// stage and line length over many grids): its half spectrum, its real field,
+// its 1-D tables (xk2, tx, yk2, ty, zk2, tz), sizes and 1 / (nx ny nz), and
+// t0, its first tile in the group's concatenated tile sequence.  A group's
+// records end with one holding only t0 = the group's tile count.
+struct PoisGrid {
+  void* spec;
+  void* rhs;
+  const double* ktab;
+  long long t0;
+  int nx, ny, nz, pad;
+  double inv_ntot;
+};
 
 // ktab layout: [xk2 nx][Tx nx][yk2 ny][Ty ny][zk2 nz][Tz nz]
 __global__ void k_spectral(double2* __restrict__ spec, const double* __restrict__ ktab, int nx,
@@ -381,10 +398,11 @@ constexpr size_t spl_lds_bytes()
          (MODE == 0 ? (size_t)2 * N * sizeof(double) : 0);
 }
 
-template <typename C, int N, int TI, int NT, int MODE>
+template <typename C, int N, int TI, int NT, int MODE, bool GRP = false>
 __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const double2* __restrict__ tw,
                                                    const double* __restrict__ ktab, int nx, int ny, int nz,
-                                                   double inv_ntot)
+                                                   double inv_ntot, const PoisGrid* __restrict__ grids,
+                                                   const int* __restrict__ bgrid, long long gtiles)
 {
 #pragma clang fp contract(off)
   static_assert(NT % TI == 0, "a thread keeps one column");
@@ -404,67 +422,142 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
   C* buf = reinterpret_cast<C*>(smem_raw);
   double2* twl = reinterpret_cast<double2*>(smem_raw + sizeof(C) * NE);
   double* kz = reinterpret_cast<double*>(twl + NTW);  // MODE 0: zk2[0..N), then tz[0..N)
-  const int nxh = nx / 2 + 1;
-  const int tiles_x = (nxh + TI - 1) / TI;
-  const int ntiles = tiles_x * (MODE == 0 ? ny : nz);
-  const long long S = MODE == 0 ? (long long)nxh * ny : (long long)nxh;   // between line points
-  const long long OS = MODE == 0 ? (long long)nxh : (long long)nxh * ny;  // between outer indices
   const int tid = threadIdx.x, col = tid % TI;
-  const double* xk2 = ktab;
-  const double* tx = xk2 + nx;
-  const double* yk2 = tx + nx;
-  const double* ty = yk2 + ny;
-  const double* zk2 = ty + ny;
-  const double* tz = zk2 + nz;
-  for (int e = tid; e < NTW; e += NT) twl[e] = tw[e];
-  if constexpr (MODE == 0) {
+  // the z tables of a grid into LDS (zk2, tz follow xk2, tx, yk2, ty in its
+  // table block)
+  auto load_kz = [&](const double* k, int gnx, int gny) {
+    const double* zk2 = k + 2 * gnx + 2 * gny;
     for (int e = tid; e < N; e += NT) {
       kz[e] = zk2[e];
-      kz[N + e] = tz[e];
+      kz[N + e] = zk2[N + e];
     }
-  }
+  };
+  for (int e = tid; e < NTW; e += NT) twl[e] = tw[e];
+  if constexpr (MODE == 0 && !GRP) load_kz(ktab, nx, ny);
   // The tables are written by other wavefronts' threads: every wavefront
   // waits for them here, since the 512-point path's first butterfly reads
   // the twiddles before the tile loop's first barrier.
   __syncthreads();
+  // A tile of one grid: TI columns from i0 of outer line index o (y for
+  // MODE 0, z else), S elements between line points.  Grouped (GRP), the
+  // tile is one of the grids' concatenated tile sequences and g its grid.
   // Element e = tid + NT q of a tile is point tid / TI + (NT / TI) q of
   // column col: a uniform row address plus this thread's 32-bit byte offset
-  // (the host checks (NT / TI) S sizeof(C) < 4 GiB), so that no 64-bit
+  // voff (the host checks (NT / TI) S sizeof(C) < 4 GiB), so that no 64-bit
   // address stays live per element.
-  const uint32_t voff = ((uint32_t)(tid / TI) * (uint32_t)S + (uint32_t)col) * (uint32_t)sizeof(C);
-  auto at = [&](long long base, int q) -> C& {
-    C* row = spec + base + (long long)(NT / TI) * q * S;
-    return *reinterpret_cast<C*>(reinterpret_cast<char*>(row) + voff);
+  struct Geo {
+    C* p;  // column i0 of line point 0
+    long long S;
+    uint32_t voff;
+    int i, o, nx, ny;
+    bool in;
+    const double* k;
+    double inv;
+  };
+  // (one grid: the constants are computed once, the tile index is an int)
+  const int nxh1 = nx / 2 + 1, tiles_x1 = (nxh1 + TI - 1) / TI;
+  const long long S1 = MODE == 0 ? (long long)nxh1 * ny : (long long)nxh1;
+  const long long OS1 = MODE == 0 ? (long long)nxh1 : (long long)nxh1 * ny;
+  const uint32_t voff1 = ((uint32_t)(tid / TI) * (uint32_t)S1 + (uint32_t)col) * (uint32_t)sizeof(C);
+  auto geo = [&](long long tile, int g) {
+    Geo G;
+    if constexpr (GRP) {
+      const PoisGrid& R = grids[g];
+      const long long lt = tile - R.t0;
+      G.k = R.ktab;
+      G.nx = R.nx;
+      G.ny = R.ny;
+      G.inv = R.inv_ntot;
+      const int nxh = G.nx / 2 + 1, tiles_x = (nxh + TI - 1) / TI;
+      G.o = (int)(lt / tiles_x);
+      const int i0 = (int)(lt - (long long)G.o * tiles_x) * TI;
+      G.S = MODE == 0 ? (long long)nxh * G.ny : (long long)nxh;
+      const long long OS = MODE == 0 ? (long long)nxh : (long long)nxh * G.ny;
+      G.p = reinterpret_cast<C*>(R.spec) + (long long)G.o * OS + i0;
+      G.voff = ((uint32_t)(tid / TI) * (uint32_t)G.S + (uint32_t)col) * (uint32_t)sizeof(C);
+      G.i = i0 + col;
+      G.in = G.i < nxh;
+    } else {
+      G.k = ktab;
+      G.nx = nx;
+      G.ny = ny;
+      G.inv = inv_ntot;
+      const int t = (int)tile;
+      G.o = t / tiles_x1;
+      const int i0 = (t - G.o * tiles_x1) * TI;
+      G.S = S1;
+      G.p = spec + (long long)G.o * OS1 + i0;
+      G.voff = voff1;
+      G.i = i0 + col;
+      G.in = G.i < nxh1;
+    }
+    return G;
+  };
+  auto at = [&](const Geo& G, int q) -> C& {
+    C* row = G.p + (long long)(NT / TI) * q * G.S;
+    return *reinterpret_cast<C*>(reinterpret_cast<char*>(row) + G.voff);
   };
   C P[E];
-  auto fetch = [&](int tile) {
-    const int o = tile / tiles_x, i0 = (tile - o * tiles_x) * TI;
-    const long long base = (long long)o * OS + i0;
-    const bool in = i0 + col < nxh;
+  auto fetch = [&](long long tile, int g) {
+    const Geo G = geo(tile, g);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q;
       P[q].x = 0;
       P[q].y = 0;
-      if (e < NE && in) P[q] = at(base, q);
+      if (e < NE && G.in) P[q] = at(G, q);
     }
   };
-  int tile = blockIdx.x;
-  if (tile < ntiles) fetch(tile);
-  for (; tile < ntiles; tile += gridDim.x) {
-    const int o = tile / tiles_x, i0 = (tile - o * tiles_x) * TI, i = i0 + col;
-    const long long base = (long long)o * OS + i0;
-    const bool in = i < nxh;
+  // One grid: work-group b walks tiles b, b + gridDim.x, ...  Grouped: a
+  // contiguous run of tiles per work-group, starting in grid bgrid[b]; the
+  // next tile is in the same grid or the next one (every grid has tiles).
+  long long tile, tend, step;
+  int g = 0;
+  if constexpr (GRP) {
+    const long long per = (gtiles + gridDim.x - 1) / gridDim.x;
+    tile = (long long)blockIdx.x * per;
+    tend = tile + per < gtiles ? tile + per : gtiles;
+    step = 1;
+    g = bgrid[blockIdx.x];
+  } else {
+    tile = blockIdx.x;
+    tend = (long long)tiles_x1 * (MODE == 0 ? ny : nz);
+    step = gridDim.x;
+  }
+  auto grid_of = [&](long long t) {
+    if constexpr (GRP) return t >= grids[g + 1].t0 ? g + 1 : g;
+    else return 0;
+  };
+  int gk = -1;  // grouped MODE 0: the grid whose z tables are in LDS
+  if (tile < tend) fetch(tile, g);
+  for (; tile < tend; tile += step) {
+    g = grid_of(tile);
+    const Geo G = geo(tile, g);
+    const bool in = G.in;
+    if constexpr (GRP && MODE == 0) {
+      // (uniform; the last readers of the previous tables passed the
+      // previous tile's closing barrier, the next readers wait for the
+      // barrier after the tile's LDS fill)
+      if (g != gk) {
+        load_kz(G.k, G.nx, G.ny);
+        gk = g;
+      }
+    }
     // this tile's x and y factors, issued before the next prefetch
     double kxx = 0, txx = 0, kyy = 0, tyy = 0;
     if constexpr (MODE == 0) {
+      const double* xk2 = G.k;
+      const double* tx = xk2 + G.nx;
+      const double* yk2 = tx + G.nx;
+      const double* ty = yk2 + G.ny;
       if (in) {
-        kxx = xk2[i];
-        txx = tx[i];
+        kxx = xk2[G.i];
+        txx = tx[G.i];
       }
-      kyy = yk2[o];
-      tyy = ty[o];
+      kyy = yk2[G.o];
+      tyy = ty[G.o];
     }
+    const bool more = tile + step < tend;
     if constexpr (REG) {
       // radix-8 passes with Ns = 1, 8, 64; thread = butterfly jj of column col
       const int jj = tid / TI;
@@ -476,7 +569,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
 #pragma unroll
       for (int r = 0; r < 8; ++r) buf[(8 * jj + r) * TI + col] = v[r];
       // the prefetch once v is in LDS, so that the two are not live together
-      if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+      if (more) fetch(tile + step, grid_of(tile + step));
       __syncthreads();
       stockham_pass<C, N, 8, TI, NT>(buf, twl, 8, inv0);
 #pragma unroll
@@ -496,7 +589,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
             v[r].y = 0;
           } else {
             using R = typename CplxT<C>::R;
-            const double qd = inv_ntot / (-K);
+            const double qd = G.inv / (-K);
             v[r].x = (R)((double)v[r].x * qd);
             v[r].y = (R)((double)v[r].y * qd);
           }
@@ -513,7 +606,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
       }
       if (in) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) at(base, r) = v[r];  // point jj + 64 r
+        for (int r = 0; r < 8; ++r) at(G, r) = v[r];  // point jj + 64 r
       }
       // buf is refilled by the next tile
       __syncthreads();
@@ -525,7 +618,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
       if (e < NE) buf[e] = P[q];
     }
     __syncthreads();
-    if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+    if (more) fetch(tile + step, grid_of(tile + step));
     if constexpr (MODE == 0) line_dft<C, N, TI, NT>(buf, twl, false);
     if constexpr (MODE == 0) {
       // (rolled loops below: their state is a few registers, the prefetch
@@ -544,7 +637,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
           v.y = 0;
         } else {
           using R = typename CplxT<C>::R;
-          const double qd = inv_ntot / (-K);
+          const double qd = G.inv / (-K);
           v.x = (R)((double)v.x * qd);
           v.y = (R)((double)v.y * qd);
         }
@@ -556,7 +649,7 @@ __global__ __launch_bounds__(NT) void k_spec_lines(C* __restrict__ spec, const d
 #pragma unroll 1
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q;
-      if (e < NE && in) at(base, q) = buf[e];
+      if (e < NE && in) at(G, q) = buf[e];
     }
     // buf is refilled by the next tile
     __syncthreads();
@@ -609,9 +702,11 @@ __device__ __forceinline__ int2 bfp_pair_word(double2 v)
 // +gridDim.x, ...  Loads and stores run along the lines (coalesced), and the
 // column layout in LDS, buf[point * ST + line], has the odd row stride
 // ST = TI + 1 so that those accesses spread over the banks.
-template <typename C, int M, int TI, int NT, bool BFP = false>
+template <typename C, int M, int TI, int NT, bool BFP = false, bool GRP = false>
 __global__ __launch_bounds__(NT) void k_rfft_x(const void* __restrict__ rhs, C* __restrict__ spec,
-                                               const double2* __restrict__ tw, long long nlines)
+                                               const double2* __restrict__ tw, long long nlines,
+                                               const PoisGrid* __restrict__ grids, const int* __restrict__ bgrid,
+                                               long long gtiles)
 {
 #pragma clang fp contract(off)
   using R = typename CplxT<C>::R;
@@ -629,26 +724,60 @@ __global__ __launch_bounds__(NT) void k_rfft_x(const void* __restrict__ rhs, C* 
   // (BFP32: the words of two points per lane; decoded when stored to LDS,
   // where the loads are waited on anyway)
   using PT = typename std::conditional<BFP, int2, C>::type;
-  const PT* in = reinterpret_cast<const PT*>(rhs);
-  const long long ntiles = (nlines + TI - 1) / TI;
+  // A tile: TI lines from l0 of one grid with nl lines (grouped: of grid g)
+  struct Geo {
+    const PT* in;
+    C* spec;
+    long long l0, nl;
+  };
+  auto geo = [&](long long tile, int g) {
+    Geo G{reinterpret_cast<const PT*>(rhs), spec, tile * TI, nlines};
+    if constexpr (GRP) {
+      const PoisGrid& Rg = grids[g];
+      G.in = reinterpret_cast<const PT*>(Rg.rhs);
+      G.spec = reinterpret_cast<C*>(Rg.spec);
+      G.l0 = (tile - Rg.t0) * TI;
+      G.nl = (long long)Rg.ny * Rg.nz;
+    }
+    return G;
+  };
   // the next tile's points are loaded into registers while this one is
   // transformed (the transforms read LDS only, so they do not wait on them)
   constexpr int E = (NE + NT - 1) / NT;
   PT P[E];
-  auto fetch = [&](long long tile) {
-    const long long l0 = tile * TI;
+  auto fetch = [&](long long tile, int g) {
+    const Geo G = geo(tile, g);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q, lt = e / M, j = e - lt * M;
       P[q].x = 0;
       P[q].y = 0;
-      if (e < NE && l0 + lt < nlines) P[q] = in[(l0 + lt) * M + j];
+      if (e < NE && G.l0 + lt < G.nl) P[q] = G.in[(G.l0 + lt) * M + j];
     }
   };
-  long long tile = blockIdx.x;
-  if (tile < ntiles) fetch(tile);
-  for (; tile < ntiles; tile += gridDim.x) {
-    const long long l0 = tile * TI;
+  // one grid: work-group b walks tiles b, b + gridDim.x, ...; grouped: a
+  // contiguous run of tiles from grid bgrid[b] on
+  long long tile, tend, step;
+  int g = 0;
+  if constexpr (GRP) {
+    const long long per = (gtiles + gridDim.x - 1) / gridDim.x;
+    tile = (long long)blockIdx.x * per;
+    tend = tile + per < gtiles ? tile + per : gtiles;
+    step = 1;
+    g = bgrid[blockIdx.x];
+  } else {
+    tile = blockIdx.x;
+    tend = (nlines + TI - 1) / TI;
+    step = gridDim.x;
+  }
+  auto grid_of = [&](long long t) {
+    if constexpr (GRP) return t >= grids[g + 1].t0 ? g + 1 : g;
+    else return 0;
+  };
+  if (tile < tend) fetch(tile, g);
+  for (; tile < tend; tile += step) {
+    g = grid_of(tile);
+    const Geo G = geo(tile, g);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q, lt = e / M, j = e - lt * M;
@@ -659,11 +788,11 @@ __global__ __launch_bounds__(NT) void k_rfft_x(const void* __restrict__ rhs, C* 
     }
     // (the first barrier also orders the twiddle copy before its readers)
     __syncthreads();
-    if (tile + (long long)gridDim.x < ntiles) fetch(tile + gridDim.x);
+    if (tile + step < tend) fetch(tile + step, grid_of(tile + step));
     line_dft<C, M, TI, NT, ST>(buf, twl, false);
     for (int e = tid; e < NH * TI; e += NT) {
       const int lt = e / NH, k = e - lt * NH;
-      if (l0 + lt >= nlines) continue;
+      if (G.l0 + lt >= G.nl) continue;
       const C a = buf[(k == M ? 0 : k) * ST + lt];
       const C b = buf[(k == 0 ? 0 : M - k) * ST + lt];  // Z_{M-k}, unconjugated
       const R ex = (a.x + b.x) * (R)0.5, ey = (a.y - b.y) * (R)0.5;
@@ -673,7 +802,7 @@ __global__ __launch_bounds__(NT) void k_rfft_x(const void* __restrict__ rhs, C* 
       C x;
       x.x = ex + tyy;
       x.y = ey - txx;
-      spec[(l0 + lt) * NH + k] = x;
+      G.spec[(G.l0 + lt) * NH + k] = x;
     }
     // buf is refilled by the next tile
     __syncthreads();
@@ -689,9 +818,11 @@ __global__ __launch_bounds__(NT) void k_rfft_x(const void* __restrict__ rhs, C* 
 // X_k, X_{M-k} (k <= M/2: lanes read one run upwards and one downwards, both
 // coalesced) and folds it in registers into Z_k and Z_{M-k}.  Tiles, LDS
 // layout, persistence and prefetch as k_rfft_x.
-template <typename C, int M, int TI, int NT, bool BFP = false>
+template <typename C, int M, int TI, int NT, bool BFP = false, bool GRP = false>
 __global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, void* __restrict__ rhs,
-                                                const double2* __restrict__ tw, long long nlines)
+                                                const double2* __restrict__ tw, long long nlines,
+                                                const PoisGrid* __restrict__ grids, const int* __restrict__ bgrid,
+                                                long long gtiles)
 {
 #pragma clang fp contract(off)
   using R = typename CplxT<C>::R;
@@ -711,29 +842,63 @@ __global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, void
   // wavefront waits here for the threads that wrote the tables
   __syncthreads();
   using OT = typename std::conditional<BFP, int2, C>::type;  // BFP32: two words per lane
-  OT* out = reinterpret_cast<OT*>(rhs);
-  const long long ntiles = (nlines + TI - 1) / TI;
+  // A tile: TI lines from l0 of one grid with nl lines (grouped: of grid g)
+  struct Geo {
+    const C* spec;
+    OT* out;
+    long long l0, nl;
+  };
+  auto geo = [&](long long tile, int g) {
+    Geo G{spec, reinterpret_cast<OT*>(rhs), tile * TI, nlines};
+    if constexpr (GRP) {
+      const PoisGrid& Rg = grids[g];
+      G.spec = reinterpret_cast<const C*>(Rg.spec);
+      G.out = reinterpret_cast<OT*>(Rg.rhs);
+      G.l0 = (tile - Rg.t0) * TI;
+      G.nl = (long long)Rg.ny * Rg.nz;
+    }
+    return G;
+  };
   constexpr int E = (NEP + NT - 1) / NT;
   C PA[E], PB[E];  // X_k, X_{M-k}
-  auto fetch = [&](long long tile) {
-    const long long l0 = tile * TI;
+  auto fetch = [&](long long tile, int g) {
+    const Geo G = geo(tile, g);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q, lt = e / NP, kp = e - lt * NP;
       PA[q].x = 0;
       PA[q].y = 0;
       PB[q] = PA[q];
-      if (e < NEP && l0 + lt < nlines) {
-        const C* row = spec + (l0 + lt) * NH;
+      if (e < NEP && G.l0 + lt < G.nl) {
+        const C* row = G.spec + (G.l0 + lt) * NH;
         PA[q] = row[kp];
         PB[q] = row[M - kp];
       }
     }
   };
-  long long tile = blockIdx.x;
-  if (tile < ntiles) fetch(tile);
-  for (; tile < ntiles; tile += gridDim.x) {
-    const long long l0 = tile * TI;
+  // one grid: work-group b walks tiles b, b + gridDim.x, ...; grouped: a
+  // contiguous run of tiles from grid bgrid[b] on
+  long long tile, tend, step;
+  int g = 0;
+  if constexpr (GRP) {
+    const long long per = (gtiles + gridDim.x - 1) / gridDim.x;
+    tile = (long long)blockIdx.x * per;
+    tend = tile + per < gtiles ? tile + per : gtiles;
+    step = 1;
+    g = bgrid[blockIdx.x];
+  } else {
+    tile = blockIdx.x;
+    tend = (nlines + TI - 1) / TI;
+    step = gridDim.x;
+  }
+  auto grid_of = [&](long long t) {
+    if constexpr (GRP) return t >= grids[g + 1].t0 ? g + 1 : g;
+    else return 0;
+  };
+  if (tile < tend) fetch(tile, g);
+  for (; tile < tend; tile += step) {
+    g = grid_of(tile);
+    const Geo G = geo(tile, g);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int e = tid + NT * q, lt = e / NP, kp = e - lt * NP;
@@ -766,7 +931,7 @@ __global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, void
       }
     }
     __syncthreads();
-    if (tile + (long long)gridDim.x < ntiles) fetch(tile + gridDim.x);
+    if (tile + step < tend) fetch(tile + step, grid_of(tile + step));
     line_dft<C, M, TI, NT, ST>(buf, twl, true);
     // (BFP32: every lane of a segment encodes; M TI is a multiple of 64)
     for (int e = tid; e < M * TI; e += NT) {
@@ -774,7 +939,7 @@ __global__ __launch_bounds__(NT) void k_irfft_x(const C* __restrict__ spec, void
       OT w;
       if constexpr (BFP) w = bfp_pair_word(buf[j * ST + lt]);
       else w = buf[j * ST + lt];
-      if (l0 + lt < nlines) out[(l0 + lt) * M + j] = w;
+      if (G.l0 + lt < G.nl) G.out[(G.l0 + lt) * M + j] = w;
     }
     // buf is refilled by the next tile
     __syncthreads();
@@ -862,7 +1027,8 @@ static int launch_spl(x3d_ctx* c, C* spec, const void* twdev)
   const unsigned blocks = (unsigned)(ntiles < cap ? ntiles : cap);
   const double inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
   hipLaunchKernelGGL((k_spec_lines<C, N, TI, NT, MODE>), dim3(blocks), dim3(NT), smem, c->stream, spec,
-                     reinterpret_cast<const double2*>(twdev), c->ktab, c->nx, c->ny, c->nz, inv_ntot);
+                     reinterpret_cast<const double2*>(twdev), c->ktab, c->nx, c->ny, c->nz, inv_ntot,
+                     (const PoisGrid*)nullptr, (const int*)nullptr, 0LL);
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
@@ -976,10 +1142,12 @@ static int rfx_run(x3d_ctx* c, void* rhs, C* spec, bool prepare)
   const long long blocks = ntiles < cap ? ntiles : cap;
   if constexpr (INV)
     hipLaunchKernelGGL((k_irfft_x<C, M, TI, NT, BFP>), dim3((unsigned)blocks), dim3(NT), smem, c->stream, spec, rhs,
-                       reinterpret_cast<const double2*>(c->xtw), nlines);
+                       reinterpret_cast<const double2*>(c->xtw), nlines, (const PoisGrid*)nullptr,
+                       (const int*)nullptr, 0LL);
   else
     hipLaunchKernelGGL((k_rfft_x<C, M, TI, NT, BFP>), dim3((unsigned)blocks), dim3(NT), smem, c->stream,
-                       (const void*)rhs, spec, reinterpret_cast<const double2*>(c->xtw), nlines);
+                       (const void*)rhs, spec, reinterpret_cast<const double2*>(c->xtw), nlines,
+                       (const PoisGrid*)nullptr, (const int*)nullptr, 0LL);
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
 }
@@ -1307,6 +1475,230 @@ int poisson_solve_bfp(x3d_ctx* c, int32_t* rhs)
   X3D_TRY_RC(launch_poisson_z<double2>(c, c->spec));
   X3D_TRY_RC((launch_poisson_y<double2, 2>(c, c->spec)));
   return launch_rfx<double2, true, true>(c, rhs, c->spec);
+}
+
+// ---- grouped Poisson of the batching form ----------------------------------
+// Per grid the stages are those of poisson_solve / poisson_solve_f32 with
+// every stage a line kernel: x r2c, forward y, z (forward, division,
+// inverse), inverse y, x c2r (here the line kernel in FP64 too: at batch
+// sizes one launch over many grids beats a hipFFT call per grid).  A grouped
+// launch covers every grid with the stage's line length: its work-groups
+// take contiguous runs of the grids' concatenated tiles.
+
+// The grouped kernel of a stage for line length n (x stages: n = nx / 2),
+// its threads, LDS bytes and tile width.  Lines of up to 384 points: a grid
+// with longer ones fills the GPU on its own and is solved by itself (the
+// grouped 512-point FP64 kernels would spill).
+struct GrpKernel {
+  const void* fn;
+  int nt;
+  size_t smem;
+  int ti;
+};
+
+template <typename C, int N, int MODE>
+static bool grp_spec_n(GrpKernel* k)
+{
+  if constexpr (spl_ok<C, N, MODE>()) {
+    constexpr int TI = LineTile<C, N>::TI, NT = LineTile<C, N>::NT;
+    *k = GrpKernel{(const void*)k_spec_lines<C, N, TI, NT, MODE, true>, NT, spl_lds_bytes<C, N, TI, MODE>(), TI};
+    return true;
+  }
+  return false;
+}
+
+template <typename C, int M, bool INV>
+static bool grp_rfx_m(GrpKernel* k)
+{
+  constexpr int TI = RfxTile<C, M>::TI, NT = RfxTile<C, M>::NT;
+  const void* fn = INV ? (const void*)k_irfft_x<C, M, TI, NT, false, true>
+                       : (const void*)k_rfft_x<C, M, TI, NT, false, true>;
+  *k = GrpKernel{fn, NT, rfx_lds_bytes<C, M, INV>(), TI};
+  return true;
+}
+
+template <typename C, int MODE>
+static bool grp_spec(int n, GrpKernel* k)
+{
+  switch (n) {
+#define X3D_GRP_N(NN) \
+    case NN: return grp_spec_n<C, NN, MODE>(k);
+    X3D_GRP_N(8) X3D_GRP_N(16) X3D_GRP_N(32) X3D_GRP_N(64) X3D_GRP_N(128) X3D_GRP_N(256)
+    X3D_GRP_N(24) X3D_GRP_N(40) X3D_GRP_N(48) X3D_GRP_N(80) X3D_GRP_N(96) X3D_GRP_N(160) X3D_GRP_N(192)
+    X3D_GRP_N(320) X3D_GRP_N(384)
+#undef X3D_GRP_N
+    default: return false;
+  }
+}
+
+template <typename C, bool INV>
+static bool grp_rfx(int m, GrpKernel* k)
+{
+  switch (m) {
+#define X3D_GRP_M(MM) \
+    case MM: return grp_rfx_m<C, MM, INV>(k);
+    X3D_GRP_M(8) X3D_GRP_M(16) X3D_GRP_M(32) X3D_GRP_M(64) X3D_GRP_M(128) X3D_GRP_M(256) X3D_GRP_M(24) X3D_GRP_M(40) X3D_GRP_M(48) X3D_GRP_M(80) X3D_GRP_M(96) X3D_GRP_M(160) X3D_GRP_M(192)
+    X3D_GRP_M(320) X3D_GRP_M(384)
+#undef X3D_GRP_M
+    default: return false;
+  }
+}
+
+// Stage s of grid c: its line length, kernel and tile count.
+template <typename C>
+static bool grp_stage(const x3d_ctx* c, int s, int* n, GrpKernel* k, long long* tiles)
+{
+  const int nxh = c->nx / 2 + 1;
+  bool ok = false;
+  switch (s) {
+    case 0: ok = grp_rfx<C, false>(*n = c->nx / 2, k); break;
+    case 1: ok = grp_spec<C, 1>(*n = c->ny, k); break;
+    case 2: ok = grp_spec<C, 0>(*n = c->nz, k); break;
+    case 3: ok = grp_spec<C, 2>(*n = c->ny, k); break;
+    case 4: ok = grp_rfx<C, true>(*n = c->nx / 2, k); break;
+    default: return false;
+  }
+  if (!ok) return false;
+  if (s == 0 || s == 4) {
+    *tiles = ((long long)c->ny * c->nz + k->ti - 1) / k->ti;
+  } else {
+    // the y / z kernels' 32-bit per-thread offsets span (NT / TI) rows
+    const long long S = s == 2 ? (long long)nxh * c->ny : nxh;
+    if ((long long)(k->nt / k->ti) * S * (long long)sizeof(C) >= (1LL << 32)) return false;
+    *tiles = (long long)((nxh + k->ti - 1) / k->ti) * (s == 2 ? c->ny : c->nz);
+  }
+  return true;
+}
+
+template <typename C>
+static int grp_build(x3d_batch* b)
+{
+  std::vector<int> grouped;
+  for (int i = 0; i < b->ng; ++i) {
+    const x3d_ctx* c = b->g[i];
+    bool ok = c->fft_ready && c->xsplit && c->ysplit && c->zsplit;
+    for (int s = 0; ok && s < 5; ++s) {
+      int n;
+      GrpKernel k;
+      long long t;
+      ok = grp_stage<C>(c, s, &n, &k, &t);
+    }
+    (ok ? grouped : b->psolo).push_back(i);
+  }
+  std::vector<unsigned char> host;
+  auto put = [&host](const void* p, size_t n) {
+    const size_t at = (host.size() + 255) & ~size_t(255);
+    host.resize(at + n);
+    std::memcpy(host.data() + at, p, n);
+    return at;
+  };
+  for (int s = 0; s < 5; ++s) {
+    std::map<int, std::vector<int>> groups;  // line length -> grids, in batch order
+    for (int i : grouped) {
+      int n;
+      GrpKernel k;
+      long long t;
+      grp_stage<C>(b->g[i], s, &n, &k, &t);
+      groups[n].push_back(i);
+    }
+    for (auto& kv : groups) {
+      const std::vector<int>& gi = kv.second;
+      x3d_ctx* c0 = b->g[gi[0]];
+      GrpKernel k;
+      int n;
+      long long t;
+      grp_stage<C>(c0, s, &n, &k, &t);
+      if (k.smem > 65536)
+        X3D_CHECK_HIP(hipFuncSetAttribute(k.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.smem));
+      int per_cu = 0;
+      X3D_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.fn, k.nt, k.smem));
+      std::vector<PoisGrid> rec;
+      long long total = 0;
+      for (int i : gi) {
+        x3d_ctx* c = b->g[i];
+        grp_stage<C>(c, s, &n, &k, &t);
+        PoisGrid r{};
+        r.spec = c->spec;
+        r.rhs = b->field(6, i);
+        r.ktab = c->ktab;
+        r.t0 = total;
+        r.nx = c->nx;
+        r.ny = c->ny;
+        r.nz = c->nz;
+        r.inv_ntot = 1.0 / ((double)c->nx * (double)c->ny * (double)c->nz);
+        rec.push_back(r);
+        total += t;
+      }
+      PoisGrid end{};
+      end.t0 = total;
+      rec.push_back(end);
+      // one launch wave; each work-group a contiguous run of `per` tiles
+      const long long cap = (long long)(b->ncu > 0 ? b->ncu : 1) * (per_cu > 0 ? per_cu : 1);
+      const long long blocks0 = total < cap ? total : cap;
+      const long long per = (total + blocks0 - 1) / blocks0;
+      const long long blocks = (total + per - 1) / per;
+      std::vector<int> bg((size_t)blocks);
+      int g = 0;
+      for (long long w = 0; w < blocks; ++w) {
+        while (rec[(size_t)g + 1].t0 <= w * per) ++g;
+        bg[(size_t)w] = g;
+      }
+      x3d_batch::PLaunch L{};
+      L.fn = k.fn;
+      L.blocks = (unsigned)blocks;
+      L.threads = (unsigned)k.nt;
+      L.smem = k.smem;
+      L.stage = s;
+      L.tw = (s == 0 || s == 4) ? c0->xtw : (s == 2 ? c0->ztw : c0->ytw);
+      L.grids = put(rec.data(), rec.size() * sizeof(PoisGrid));
+      L.bgrid = put(bg.data(), bg.size() * sizeof(int));
+      L.tiles = total;
+      b->plaunch.push_back(L);
+    }
+  }
+  if (!host.empty()) {
+    X3D_CHECK_HIP(hipMalloc(&b->pargs, host.size()));
+    X3D_CHECK_HIP(hipMemcpy(b->pargs, host.data(), host.size(), hipMemcpyHostToDevice));
+  }
+  return X3D_OK;
+}
+
+int poisson_batch_build(x3d_batch* b)
+{
+  if (b->pargs) (void)hipFree(b->pargs);
+  b->pargs = nullptr;
+  b->plaunch.clear();
+  b->psolo.clear();
+  const char* e = std::getenv("X3D_BATCH_PGROUP");
+  if (e && std::atoi(e) == 0) {
+    for (int i = 0; i < b->ng; ++i) b->psolo.push_back(i);
+    return X3D_OK;
+  }
+  return b->prec == 8 ? grp_build<double2>(b) : grp_build<float2>(b);
+}
+
+int poisson_batch_run(x3d_batch* b)
+{
+  for (const x3d_batch::PLaunch& L : b->plaunch) {
+    const PoisGrid* grids = reinterpret_cast<const PoisGrid*>(static_cast<char*>(b->pargs) + L.grids);
+    const int* bgrid = reinterpret_cast<const int*>(static_cast<char*>(b->pargs) + L.bgrid);
+    const double2* tw = static_cast<const double2*>(L.tw);
+    long long tiles = L.tiles;
+    // (the single-grid arguments are unused by the grouped kernels)
+    void* none = nullptr;
+    const double* ktab = nullptr;
+    int zero = 0;
+    long long nlines = 0;
+    double inv = 0.0;
+    if (L.stage == 0 || L.stage == 4) {
+      void* args[] = {&none, &none, &tw, &nlines, &grids, &bgrid, &tiles};
+      X3D_CHECK_HIP(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(L.threads), args, L.smem, b->stream));
+    } else {
+      void* args[] = {&none, &tw, &ktab, &zero, &zero, &zero, &inv, &grids, &bgrid, &tiles};
+      X3D_CHECK_HIP(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(L.threads), args, L.smem, b->stream));
+    }
+  }
+  return X3D_OK;
 }
 
 }  // namespace x3d
