@@ -10,6 +10,9 @@ over every grid the line kernels take, the other grids solved one by one.
   step's pressure comes from the O(gdt) divergence left after the first
   projection, about 1e3 times smaller, so the same rounding reads 1e3 times
   larger relative to it).
+* The launches of one pass or Poisson stage side by side on the side streams
+  (the default) against the same launches in order (X3D_BATCH_OVERLAP=0):
+  bit for bit, eager and captured.
 * A batch mixing grouped grids with ones solved alone (an x length the x line
   kernel is not built for, a 512-point z line) against the oracle at the FP64
   bar, eager and through a captured graph (replays equal to eager steps bit
@@ -93,3 +96,31 @@ def test_mixed_batch_matches_oracle_eager_and_captured(torch_cuda, monkeypatch):
     finally:
         eager.close()
         graph.close()
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_overlapped_launches_equal_serial(torch_cuda, prec, monkeypatch):
+    from x3div_amd import X3DBatch
+    print(f"\n[overlapped vs serial launches, precision {prec}]", flush=True)
+    monkeypatch.setenv("X3D_BATCH_OVERLAP", "0")
+    serial = X3DBatch(DIMS, precision=prec)
+    monkeypatch.delenv("X3D_BATCH_OVERLAP")
+    over = X3DBatch(DIMS, precision=prec)
+    captured = X3DBatch(DIMS, precision=prec)
+    try:
+        for b in (serial, over, captured):
+            b.init_fields(kind=2, seed=SEED)
+        captured.graph_capture(gdt=1e-3)
+        for step in range(2):
+            serial.step(gdt=1e-3)
+            over.step(gdt=1e-3)
+            captured.graph_launch()
+            for i in range(len(DIMS)):
+                for name in FIELDS:
+                    ref = serial.get(i, name)
+                    assert np.array_equal(over.get(i, name), ref), (step, i, name)
+                    assert np.array_equal(captured.get(i, name), ref), (step, i, name)
+    finally:
+        serial.close()
+        over.close()
+        captured.close()

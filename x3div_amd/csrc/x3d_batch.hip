@@ -21,6 +21,8 @@
 
 using namespace x3d;
 
+#define X3D_TRY_B(expr) do { const int _r = (expr); if (_r) return _r; } while (0)
+
 namespace {
 
 __global__ void k_to_f32(float* __restrict__ out, const double* __restrict__ in, long long n)
@@ -46,21 +48,44 @@ int set_streams(x3d_batch* b, hipStream_t s)
   return X3D_OK;
 }
 
+}  // namespace
+
+int x3d::batch_fork(x3d_batch* b)
+{
+  X3D_CHECK_HIP(hipEventRecord(b->fork_ev, b->stream));
+  for (hipStream_t s : b->pstreams) X3D_CHECK_HIP(hipStreamWaitEvent(s, b->fork_ev, 0));
+  return X3D_OK;
+}
+
+int x3d::batch_join(x3d_batch* b, int rc)
+{
+  for (size_t k = 0; k < b->pstreams.size(); ++k) {
+    if (hipEventRecord(b->join_ev[k], b->pstreams[k]) != hipSuccess && !rc) rc = X3D_ERR_HIP;
+    if (hipStreamWaitEvent(b->stream, b->join_ev[k], 0) != hipSuccess && !rc) rc = X3D_ERR_HIP;
+  }
+  return rc;
+}
+
+namespace {
+
 // Poisson of every grid.  The grids the line kernels take run grouped: one
-// persistent launch per stage and line length over all of them, on the batch
-// stream.  The others (psolo) solve one by one; a grid's solve is a few small
-// kernels that leave most compute units idle, so they go round-robin onto the
-// side streams, forked from the batch stream and joined back to it, and run
-// beside each other and the grouped launches.  Each such context runs on its
-// side stream for the call and is given the batch stream back afterwards.
+// persistent launch per stage and line length over all of them.  With no
+// other grids, a stage's launches run side by side on the side streams.  The
+// others (psolo) solve one by one; a grid's solve is a few small kernels that
+// leave most compute units idle, so they go round-robin onto the side
+// streams, forked from the batch stream and joined back to it, and run beside
+// each other and the grouped launches (then on the batch stream).  Each such
+// context runs on its side stream for the call and is given the batch stream
+// back afterwards.
 int batch_poisson(x3d_batch* b)
 {
-  const int ns = b->psolo.empty() ? 0 : (int)b->pstreams.size();
-  if (ns > 0) {
-    X3D_CHECK_HIP(hipEventRecord(b->fork_ev, b->stream));
-    for (hipStream_t s : b->pstreams) X3D_CHECK_HIP(hipStreamWaitEvent(s, b->fork_ev, 0));
+  if (b->psolo.empty()) {
+    const int ns = b->overlap ? (int)b->pstreams.size() : 0;
+    return poisson_batch_run(b, ns);
   }
-  int rc = poisson_batch_run(b);
+  const int ns = (int)b->pstreams.size();
+  if (ns > 0) X3D_TRY_B(batch_fork(b));
+  int rc = poisson_batch_run(b, 0);
   for (size_t k = 0; !rc && k < b->psolo.size(); ++k) {
     const int i = b->psolo[k];
     x3d_ctx* c = b->g[i];
@@ -71,10 +96,7 @@ int batch_poisson(x3d_batch* b)
   if (ns > 0) {
     // join every side stream, also after an error, so that no capture or
     // later batch work is left unordered against them
-    for (int k = 0; k < ns; ++k) {
-      if (hipEventRecord(b->join_ev[k], b->pstreams[k]) != hipSuccess && !rc) rc = X3D_ERR_HIP;
-      if (hipStreamWaitEvent(b->stream, b->join_ev[k], 0) != hipSuccess && !rc) rc = X3D_ERR_HIP;
-    }
+    rc = batch_join(b, rc);
     const int rs = set_streams(b, b->stream);
     if (!rc) rc = rs;
   }
@@ -211,10 +233,9 @@ int x3d_batch_create(int ngrids, const int* dims, const double* lengths, int pre
     x3d_batch_destroy(b);
     return rc;
   }
-  const int nsolo = (int)b->psolo.size();
-  int nps = std::min(4, nsolo);
+  int nps = 4;
   if (const char* ps = std::getenv("X3D_BATCH_PSTREAMS")) nps = std::max(0, std::min(16, std::atoi(ps)));
-  nps = std::min(nps, nsolo);
+  if (const char* ov = std::getenv("X3D_BATCH_OVERLAP")) b->overlap = std::atoi(ov) != 0;
   if (nps > 0) {
     if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) {
       x3d_batch_destroy(b);

@@ -721,14 +721,13 @@ int batch_build(x3d_batch* b, double gdt)
 }
 
 template <typename T>
-static int batch_launch(x3d_batch* b, const x3d_batch::Launch& L)
+static int batch_launch(x3d_batch* b, const x3d_batch::Launch& L, hipStream_t st)
 {
   const char* base = static_cast<const char*>(b->dargs);
   const int* block0 = reinterpret_cast<const int*>(base + L.block0);
   const void* args = base + L.args;
   const TranseqArgs<T>* ta = static_cast<const TranseqArgs<T>*>(args);
   const PassArgs<T>* pa = static_cast<const PassArgs<T>*>(args);
-  hipStream_t st = b->stream;
   switch (L.pass) {
 #define X3D_BT(P, K, DIR, LIST)                                                                             \
   case P:                                                                                                 \
@@ -751,9 +750,22 @@ template <typename T>
 int batch_passes(x3d_batch* b, int which)
 {
   if (!b->built) return X3D_ERR_ARG;
-  for (int p = 3 * which; p < 3 * which + 3; ++p)
+  const int ns = b->overlap ? (int)b->pstreams.size() : 0;
+  for (int p = 3 * which; p < 3 * which + 3; ++p) {
+    std::vector<const x3d_batch::Launch*> ls;
     for (const auto& L : b->launches)
-      if (L.pass == p) X3D_TRY(batch_launch<T>(b, L));
+      if (L.pass == p) ls.push_back(&L);
+    if (ns == 0 || ls.size() < 2) {
+      for (const auto* L : ls) X3D_TRY(batch_launch<T>(b, *L, b->stream));
+      continue;
+    }
+    // the pass's groups (disjoint grids) side by side; the next pass waits
+    // for all of them
+    X3D_TRY(batch_fork(b));
+    int rc = X3D_OK;
+    for (size_t k = 0; !rc && k < ls.size(); ++k) rc = batch_launch<T>(b, *ls[k], b->pstreams[k % ns]);
+    X3D_TRY(batch_join(b, rc));
+  }
   return X3D_OK;
 }
 

@@ -181,8 +181,12 @@ struct x3d_batch {
   std::vector<PLaunch> plaunch;
   std::vector<int> psolo;
   void* pargs = nullptr;
-  // side streams of the per-grid Poisson solves (forked from and joined back
-  // to the batch stream each step), and their fork / join events
+  // side streams (forked from and joined back to the batch stream): the
+  // launches of one pass or grouped Poisson stage, which touch disjoint grids,
+  // spread over them so that each overlaps the others' ramp-up and drain
+  // (overlap; X3D_BATCH_OVERLAP=0 keeps them in order), and the per-grid
+  // Poisson solves; their fork / join events
+  bool overlap = true;
   std::vector<hipStream_t> pstreams;
   hipEvent_t fork_ev = nullptr;
   std::vector<hipEvent_t> join_ev;
@@ -200,6 +204,10 @@ template <typename T>
 int batch_build(x3d_batch* b, double gdt);
 template <typename T>
 int batch_passes(x3d_batch* b, int which);
+// The side streams wait for the batch stream's work so far (fork), the batch
+// stream for theirs (join; also after an error, returning the first one).
+int batch_fork(x3d_batch* b);
+int batch_join(x3d_batch* b, int rc);
 int line_plan(int n, bool line_fast, LinePlan* out);
 int line_plan_fused(int n, bool line_fast, LinePlan* out, int variant);
 
@@ -271,9 +279,10 @@ int poisson_solve(x3d_ctx* c, double* rhs);
 int poisson_solve_bfp(x3d_ctx* c, int32_t* rhs);
 // Grouped Poisson of a batch (x3d_poisson.hip): build the launches and their
 // device records (outside any capture; X3D_BATCH_PGROUP=0 solves every grid
-// on its own), and run them on the batch stream.
+// on its own), and run them: on the batch stream, or with ns > 0 each stage's
+// launches spread over the first ns side streams.
 int poisson_batch_build(x3d_batch* b);
-int poisson_batch_run(x3d_batch* b);
+int poisson_batch_run(x3d_batch* b, int ns);
 
 int ensure_work(x3d_ctx* c);
 

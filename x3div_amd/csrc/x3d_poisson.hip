@@ -1677,9 +1677,9 @@ int poisson_batch_build(x3d_batch* b)
   return b->prec == 8 ? grp_build<double2>(b) : grp_build<float2>(b);
 }
 
-int poisson_batch_run(x3d_batch* b)
+static int grp_launch(x3d_batch* b, const x3d_batch::PLaunch& L, hipStream_t st)
 {
-  for (const x3d_batch::PLaunch& L : b->plaunch) {
+  {
     const PoisGrid* grids = reinterpret_cast<const PoisGrid*>(static_cast<char*>(b->pargs) + L.grids);
     const int* bgrid = reinterpret_cast<const int*>(static_cast<char*>(b->pargs) + L.bgrid);
     const double2* tw = static_cast<const double2*>(L.tw);
@@ -1692,11 +1692,31 @@ int poisson_batch_run(x3d_batch* b)
     double inv = 0.0;
     if (L.stage == 0 || L.stage == 4) {
       void* args[] = {&none, &none, &tw, &nlines, &grids, &bgrid, &tiles};
-      X3D_CHECK_HIP(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(L.threads), args, L.smem, b->stream));
+      X3D_CHECK_HIP(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(L.threads), args, L.smem, st));
     } else {
       void* args[] = {&none, &tw, &ktab, &zero, &zero, &zero, &inv, &grids, &bgrid, &tiles};
-      X3D_CHECK_HIP(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(L.threads), args, L.smem, b->stream));
+      X3D_CHECK_HIP(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(L.threads), args, L.smem, st));
     }
+  }
+  return X3D_OK;
+}
+
+int poisson_batch_run(x3d_batch* b, int ns)
+{
+  for (int s = 0; s < 5; ++s) {
+    std::vector<const x3d_batch::PLaunch*> ls;
+    for (const auto& L : b->plaunch)
+      if (L.stage == s) ls.push_back(&L);
+    if (ns <= 0 || ls.size() < 2) {
+      for (const auto* L : ls) X3D_TRY_RC(grp_launch(b, *L, b->stream));
+      continue;
+    }
+    // the stage's groups (disjoint grids) side by side; the next stage waits
+    // for all of them
+    X3D_TRY_RC(batch_fork(b));
+    int rc = X3D_OK;
+    for (size_t k = 0; !rc && k < ls.size(); ++k) rc = grp_launch(b, *ls[k], b->pstreams[k % ns]);
+    X3D_TRY_RC(batch_join(b, rc));
   }
   return X3D_OK;
 }
