@@ -250,7 +250,10 @@ int x3d_step(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
  * context's stream, waits for it, and writes each pass's milliseconds to
  * pass_ms[X3D_NPASS].  Pass i is named by x3d_pass_name(i); its algorithmic
  * HBM bytes per grid point (every field it reads or writes touched once) are
- * x3d_pass_bytes_per_point(i). */
+ * x3d_pass_bytes_per_point(i), in FP64.  The transeq passes run z, y, x: z
+ * reads u and writes its part (6 fields), x adds the sum and the Euler update
+ * (12 fields).  The FP32 form's single grids run them x, y, z: there
+ * transeq_x moves 6 fields and transeq_z 12 (at 4 B each). */
 #define X3D_NPASS 10
 int x3d_step_timed(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *uz,
                    double *dux, double *duy, double *duz, double *pp3,

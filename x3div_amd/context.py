@@ -348,13 +348,17 @@ class X3D:
             self._h, float(gdt), *(self._f32(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz))))
 
     def step_timed_f32(self, ux, uy, uz, dux, duy, duz, pp3, px, py, pz, gdt: float = 0.0):
-        """step_timed for the precision form; bytes per point are half the FP64 figures."""
+        """step_timed for the precision form; bytes per point are half the FP64
+        figures, with those of transeq_x and transeq_z exchanged (FP32 single
+        grids run transeq x first, so its x pass moves 6 fields, its z pass 12)."""
         from ._lib import X3D_NPASS
         ms = (C.c_float * X3D_NPASS)()
         check("x3d_step_timed_f32", self.lib.x3d_step_timed_f32(
             self._h, float(gdt), *(self._f32(t) for t in (ux, uy, uz, dux, duy, duz, pp3, px, py, pz)), ms))
-        return {self.lib.x3d_pass_name(i).decode(): (ms[i], self.lib.x3d_pass_bytes_per_point(i) / 2)
-                for i in range(X3D_NPASS)}
+        out = {self.lib.x3d_pass_name(i).decode(): [ms[i], self.lib.x3d_pass_bytes_per_point(i) / 2]
+               for i in range(X3D_NPASS)}
+        out["transeq_x"][1], out["transeq_z"][1] = out["transeq_z"][1], out["transeq_x"][1]
+        return {k: tuple(v) for k, v in out.items()}
 
     # ---- BFP32 form (block-floating-point int32 fields, FP64 line solves) ----------
     def _bfp(self, t):

@@ -1089,7 +1089,7 @@ int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double*
     X3D_TRY(x3d_cor_vel(c, ux, uy, uz, px, py, pz));
     return X3D_OK;
   }
-  // transeq with the Euler update folded into its z pass, divergence, Poisson,
+  // transeq with the Euler update folded into its x pass, divergence, Poisson,
   // gradp with cor_vel folded into its x pass
   X3D_TRY(fused_momentum<double>(c, gdt, 1, ux, uy, uz, dux, duy, duz));
   X3D_TRY(fused_divergence<double>(c, pp3, ux, uy, uz));
@@ -1101,8 +1101,9 @@ int x3d_step(x3d_ctx* c, double gdt, double* ux, double* uy, double* uz, double*
 
 namespace {
 // Pass names and algorithmic HBM bytes per grid point (FP64: each field read or
-// written once per pass).  transeq Y reads u, du and writes tmp = du + its
-// part; transeq Z reads u, tmp and writes du, u; gradp X
+// written once per pass), in launch order.  transeq Z reads u and writes du =
+// its part; transeq Y reads u, du and writes tmp = du + its part; transeq X
+// reads u, tmp and writes du, u; gradp X
 // reads three gradients and u and writes p gradients and u.  The Poisson figure
 // is one r2c read/write, one in-place spectral update and one c2r read/write.
 struct PassInfo {
@@ -1110,7 +1111,7 @@ struct PassInfo {
   int fields;
 };
 constexpr PassInfo kPass[X3D_NPASS] = {
-    {"transeq_x", 6}, {"transeq_y", 9}, {"transeq_z", 12}, {"divergence_x", 6}, {"divergence_y", 5},
+    {"transeq_z", 6}, {"transeq_y", 9}, {"transeq_x", 12}, {"divergence_x", 6}, {"divergence_y", 5},
     {"divergence_z", 3}, {"poisson", 6}, {"gradp_z", 3}, {"gradp_y", 5}, {"gradp_x", 12}};
 }  // namespace
 
@@ -1187,6 +1188,8 @@ int x3d_step_timed_f32(x3d_ctx* c, double gdt, float* ux, float* uy, float* uz, 
   X3D_CHECK_HIP(hipEventSynchronize(c->pass_ev[X3D_NPASS]));
   for (int i = 0; i < X3D_NPASS; ++i)
     X3D_CHECK_HIP(hipEventElapsedTime(&pass_ms[i], c->pass_ev[i], c->pass_ev[i + 1]));
+  // the pass names follow the z-first transeq order
+  if (!transeq_zfirst<float>()) std::swap(pass_ms[0], pass_ms[2]);
   return X3D_OK;
 }
 

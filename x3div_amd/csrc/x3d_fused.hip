@@ -2,7 +2,8 @@
 // fields, in one sweep over the lines, with the reference's elementwise glue
 // (products, combinations, Euler update, velocity correction) folded into the
 // prologue/epilogue.  At one rank the pencils are one array, so the step is
-//   transeq   X -> Y -> Z(+combine, Euler)        src/transeq.f90:44-355
+//   transeq   Z -> Y -> X(+combine, Euler)        src/transeq.f90:44-355
+//             (FP32 single grids: X -> Y -> Z(+combine, Euler))
 //   divergence X -> Y -> Z                         src/navier.f90:98-236
 //   Poisson                                        src/poisson.f90:276-419
 //   gradp     Z -> Y -> X(+cor_vel)                src/navier.f90:249-359, 64-91
@@ -16,9 +17,18 @@
 namespace x3d {
 
 // ---- transeq: r_c = D(u_d u_c) + u_d D(u_c), d = DIR ----------------------
-// X: du_c = r_c; Y: tmp_c = du_c + r_c;
-// Z: du_c = tmp_c + r_c; if euler: u_c = gdt du_c + u_c.
-// (src/transeq.f90:100-115, 120-134, 139-153, 158-177 per component)
+// ZF (z first): Z: du_c = r_c; Y: tmp_c = du_c + r_c;
+//               X: du_c = tmp_c + r_c; if euler: u_c = gdt du_c + u_c.
+// else:         X: du_c = r_c; Y: tmp_c = du_c + r_c;
+//               Z: du_c = tmp_c + r_c; if euler: u_c = gdt du_c + u_c.
+// (src/transeq.f90:100-115, 120-134, 139-153, 158-177 per component; the
+// reference sums the x, y, z parts in that order, ZF the z, y, x parts: the
+// same terms, rounded in another order.)  With ZF the z pass, whose strided
+// rows cost an address translation each, reads u and writes one part (6
+// fields), and the x pass, whose rows are contiguous, carries the sums and
+// the Euler update (12 fields).  Measured at 512^3, that is faster for FP64
+// (6.81 -> 6.75 ms for the three passes) and BFP32 (6.81 -> 5.86 ms), slower
+// for FP32 (3.11 -> 3.21 ms), whose single grids keep x first.
 // S: the fields' storage (FP64, FP32, or BFP32 computed in FP64).
 template <typename S>
 struct TranseqArgs {
@@ -33,9 +43,12 @@ struct TranseqArgs {
   const T* sy;  // y-stretching ppy(j), or nullptr (y pass only)
 };
 
-template <typename S, int M, int DIR>
+template <typename S, int M, int DIR, bool ZF>
 __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ A, int blk)
 {
+  // this pass's place in the order: the first writes du_c, the last adds
+  // tmp_c and applies the Euler update
+  constexpr bool FIRST = DIR == (ZF ? 2 : 0), LAST = DIR == (ZF ? 0 : 2);
   using T = typename Store<S>::T;
   S* const ux = A.ux; S* const uy = A.uy; S* const uz = A.uz;
   S* const dux = A.dux; S* const duy = A.duy; S* const duz = A.duz;
@@ -59,11 +72,11 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
     xd = x_block<S, M>(geo, t, lds + (blockDim.x >> 6) * x_tile_words<M>());
     x_stage_in<S, M>(Ud, xd);
   }
-  // y/z: the u_d window (the same for the three components) and, with the
-  // Euler update, each u_c chunk are kept in per-thread LDS slots from their
-  // loads to their later uses instead of being loaded again: strided loads
-  // cost address translation per row, LDS slots do not.  Slot q of thread i
-  // is keep[q * blockDim.x + i], after the solver's words.
+  // y/z: the u_d window (the same for the three components) and, for the
+  // Euler update of a last z pass, each u_c chunk are kept in per-thread LDS
+  // slots from their loads to their later uses instead of being loaded again:
+  // strided loads cost address translation per row, LDS slots do not.  Slot
+  // q of thread i is keep[q * blockDim.x + i], after the solver's words.
   T* const keep = lds + line_lds_words(t.Wc, t.LPW);
   const int nth = blockDim.x, tid = threadIdx.x;
 #pragma unroll
@@ -102,7 +115,7 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
         } else {
           load_window<S, M, H, H, true>(Uc, t, n, wc);
         }
-        if constexpr (DIR == 2) {
+        if constexpr (DIR == 2 && LAST) {
 #pragma unroll
           for (int k = 0; k < M; ++k) keep[(NW + k) * nth + tid] = wc[H + k];
         }
@@ -115,15 +128,15 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
                 cf.b * (wd[q + 2] * wc[q + 2] - wd[q - 2] * wc[q - 2]);
       }
     }
-    // y/z: the epilogue's strided loads (du_c from the x pass, resp. tmp_c)
-    // are issued here and complete during the two line solves, whose
-    // barriers wait on LDS only.  Neither field is written by this pass.
-    // (not for FP32 z-lines of 32-point chunks, which would spill)
-    constexpr bool PF = DIR != 0 && !(sizeof(T) == 4 && M >= 32 && DIR == 2);
+    // y, and a last z pass: the epilogue's strided loads (du_c of the first
+    // pass, resp. tmp_c) are issued here and complete during the two line
+    // solves, whose barriers wait on LDS only.  Neither field is written by
+    // this pass.  (not for FP32 z-lines of 32-point chunks, which would spill)
+    constexpr bool PF = DIR == 1 || (DIR == 2 && LAST && !(sizeof(T) == 4 && M >= 32));
     T a[M];
     if constexpr (PF) {
       if (t.active) {
-        const S* src = DIR == 1 ? DUc : TMc;
+        const S* src = DIR == 1 ? DUc : TMc;  // (DIR 2: tmp_c)
 #pragma unroll
         for (int k = 0; k < M; ++k) a[k] = sld(src, t.at(t.j0 + k));
       }
@@ -143,11 +156,30 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
 #pragma unroll
       for (int k = 0; k < M; ++k) v1[k] = v1[k] + keep[(H + k) * nth + tid] * v2[k];
     }
-    if constexpr (DIR == 0) {
+    if constexpr (DIR == 0 && FIRST) {
       x_store<S, M>(DUc, xb, v1);
+    } else if constexpr (DIR == 0) {
+      // du_c = tmp_c + r_c and the Euler update, through the staging tile:
+      // u_c at the chunk points first (the u_x tile, resp. the tile still
+      // holding this component's window load), then tmp_c restaged
+      T uc[M], a[M];
+      {
+        const T* own = (c == 0 ? xd.tile + xd.lane * XPitch<M>::P : xb.tile + xb.lane * XPitch<M>::P);
+#pragma unroll
+        for (int k = 0; k < M; ++k) uc[k] = own[k];
+      }
+      x_load_chunk<S, M>(TMc, xb, a);
+#pragma unroll
+      for (int k = 0; k < M; ++k) v1[k] = a[k] + v1[k];
+      x_store<S, M>(DUc, xb, v1);
+      if (euler) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) v1[k] = gdt * v1[k] + uc[k];
+        x_store<S, M>(Uc, xb, v1);
+      }
     } else if constexpr (DIR == 1) {
-      // tmp_c = du_c (x part) + r_c: the sum is formed here, where a strided
-      // row stays inside one page, rather than in the z pass
+      // tmp_c = du_c (first pass's part) + r_c: the sum is formed here, where
+      // a strided row stays inside one page, rather than in a last z pass
       if (A.sy) {
 #pragma unroll
         for (int k = 0; k < M; ++k) v1[k] = v1[k] * A.sy[t.j0 + k];
@@ -155,6 +187,11 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
       if (t.active) {
 #pragma unroll
         for (int k = 0; k < M; ++k) sst(TMc, t.at(t.j0 + k), a[k] + v1[k]);
+      }
+    } else if constexpr (FIRST) {
+      if (t.active) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) sst(DUc, t.at(t.j0 + k), v1[k]);
       }
     } else {
       if (t.active) {
@@ -367,8 +404,8 @@ __device__ __forceinline__ int grid_of(const int* __restrict__ block0, int ngrid
   return lo;
 }
 
-template <typename S, int M, int DIR>
-__global__ __launch_bounds__(512) void k_transeq(TranseqArgs<S> A) { transeq_body<S, M, DIR>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
+template <typename S, int M, int DIR, bool ZF>
+__global__ __launch_bounds__(512) void k_transeq(TranseqArgs<S> A) { transeq_body<S, M, DIR, ZF>(A, A.tl.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x); }
 // y passes: at most 128 VGPRs, so that two work-groups share a compute unit
 // (measured faster at 512^3 in both precisions despite a few spills)
 template <typename S, int M, int DIR>
@@ -383,7 +420,7 @@ __global__ __launch_bounds__(512) void k_transeq_batch(const TranseqArgs<T>* __r
   const int total = block0[ngrids];
   for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
     const int g = grid_of(block0, ngrids, vb);
-    transeq_body<T, M, DIR>(list[g], vb - block0[g]);
+    transeq_body<T, M, DIR, true>(list[g], vb - block0[g]);
   }
 }
 template <typename T, int M, int DIR>
@@ -451,10 +488,10 @@ static DirSet<S> dirset(const x3d_ctx* c, int dir)
   if constexpr (is_bfp<S>)
     if (dir == 2) s.tl.xcd = c->xcd_remap;
   s.M = P.M;
-  // keep slots of transeq: the u_d window (M + 4) and, along z, the u_c chunk
+  // keep slots of transeq: the u_d window (M + 4)
   s.smem_tq = s.smem;
   if (P.line_fast)
-    s.smem_tq += sizeof(T) * (size_t)s.block.x * (size_t)(P.M + 4 + (dir == 2 ? P.M : 0));
+    s.smem_tq += sizeof(T) * (size_t)s.block.x * (size_t)(P.M + 4);
   else
     s.smem_tq = 2 * s.smem;  // x: the staging tile and the u_x tile
   const Scheme& sc = c->sc[dir];
@@ -529,9 +566,12 @@ static int transeq_pass(x3d_ctx* c, S* ux, S* uy, S* uz, S* dux, S* duy, S* duz,
   const TranseqArgs<S> A{ux,   uy,   uz,   dux,  duy,      duz,   tx,
                          ty,   tz,   s.cf_d, s.cc_d, s.geo, s.tl, (T)gdt,
                          euler, DIR == 1 ? stretch_vec<T>(c, 0) : nullptr};
+  // (a last z pass keeps each u_c chunk for the Euler update as well)
+  constexpr bool ZF = transeq_zfirst<S>();
+  const size_t smem = s.smem_tq + (DIR == 2 && !ZF ? sizeof(T) * (size_t)s.block.x * (size_t)s.M : 0);
   X3D_MSWITCH_T(S, s.M, {
-    allow_lds((const void*)k_transeq<S, MM, DIR>, s.smem_tq);
-    hipLaunchKernelGGL((k_transeq<S, MM, DIR>), s.grid, s.block, s.smem_tq, c->stream, A);
+    allow_lds((const void*)k_transeq<S, MM, DIR, ZF>, smem);
+    hipLaunchKernelGGL((k_transeq<S, MM, DIR, ZF>), s.grid, s.block, smem, c->stream, A);
   });
   X3D_CHECK_HIP(hipGetLastError());
   return X3D_OK;
@@ -584,11 +624,12 @@ int fused_momentum(x3d_ctx* c, double gdt, int euler, T* ux, T* uy, T* uz, T* du
   if (sizeof(T) == 8 && !c->f64_lines) return X3D_ERR_UNSUPPORTED;
   X3D_TRY(ensure_work(c));
   T *tx = wk<T>(c, 5), *ty = wk<T>(c, 6), *tz = wk<T>(c, 7);
-  X3D_TRY((transeq_pass<0, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0)));
+  constexpr int D0 = transeq_zfirst<T>() ? 2 : 0;  // first pass; the last is 2 - D0
+  X3D_TRY((transeq_pass<D0, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0)));
   X3D_TRY(mark_pass(c, 1));
   X3D_TRY((transeq_pass<1, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, 0)));
   X3D_TRY(mark_pass(c, 2));
-  X3D_TRY((transeq_pass<2, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, euler)));
+  X3D_TRY((transeq_pass<2 - D0, T>(c, ux, uy, uz, dux, duy, duz, tx, ty, tz, gdt, euler)));
   X3D_TRY(mark_pass(c, 3));
   return X3D_OK;
 }
@@ -632,7 +673,7 @@ struct PlanKey {
     return M != o.M ? M < o.M : G != o.G ? G < o.G : Wc != o.Wc ? Wc < o.Wc : Wl != o.Wl ? Wl < o.Wl : LW < o.LW;
   }
 };
-constexpr int kPassDir[9] = {0, 1, 2, 0, 1, 2, 2, 1, 0};
+constexpr int kPassDir[9] = {2, 1, 0, 0, 1, 2, 2, 1, 0};
 }  // namespace
 
 template <typename T>
@@ -736,7 +777,7 @@ static int batch_launch(x3d_batch* b, const x3d_batch::Launch& L, hipStream_t st
       hipLaunchKernelGGL((K<T, MM, DIR>), L.grid, L.block, L.smem, st, LIST, block0, L.ngroup);           \
     });                                                                                                   \
     break;
-    X3D_BT(0, k_transeq_batch, 0, ta) X3D_BT(1, k_transeq_batch, 1, ta) X3D_BT(2, k_transeq_batch, 2, ta)
+    X3D_BT(0, k_transeq_batch, 2, ta) X3D_BT(1, k_transeq_batch, 1, ta) X3D_BT(2, k_transeq_batch, 0, ta)
     X3D_BT(3, k_div_batch, 0, pa) X3D_BT(4, k_div_batch, 1, pa) X3D_BT(5, k_div_batch, 2, pa)
     X3D_BT(6, k_grad_batch, 2, pa) X3D_BT(7, k_grad_batch, 1, pa) X3D_BT(8, k_grad_batch, 0, pa)
 #undef X3D_BT

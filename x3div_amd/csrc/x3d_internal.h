@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/x3dhip.h"
@@ -149,7 +150,7 @@ struct x3d_batch {
   // device argument records of the grouped launches, rebuilt when the plans
   // or gdt change
   struct Launch {
-    int pass;       // 0-2 transeq x y z, 3-5 divergence x y z, 6-8 gradp z y x
+    int pass;       // 0-2 transeq z y x, 3-5 divergence x y z, 6-8 gradp z y x
     int M;
     dim3 grid, block;
     size_t smem;
@@ -204,6 +205,11 @@ template <typename T>
 int batch_build(x3d_batch* b, double gdt);
 template <typename T>
 int batch_passes(x3d_batch* b, int which);
+// Order of the single-grid transeq passes for storage S: z, y, x (true), or
+// x, y, z for FP32 (measured faster there).  The batch always runs z, y, x.
+// The timed steps report the passes under the names of the z-first order.
+template <typename S>
+constexpr bool transeq_zfirst() { return !std::is_same<S, float>::value; }
 // The side streams wait for the batch stream's work so far (fork), the batch
 // stream for theirs (join; also after an error, returning the first one).
 int batch_fork(x3d_batch* b);
