@@ -79,6 +79,13 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
   // q of thread i is keep[q * blockDim.x + i], after the solver's words.
   T* const keep = lds + line_lds_words(t.Wc, t.LPW);
   const int nth = blockDim.x, tid = threadIdx.x;
+  // a first z pass: the u_y window of the second component is loaded during
+  // the first component's line solves (its strided rows wait on address
+  // translation).  FP64 16-point chunks only, whose 512-thread work-groups
+  // hold 2 waves per SIMD with or without the window's registers (shorter
+  // chunks would lose occupancy); not for BFP32, whose loads decode at once.
+  constexpr bool PREW = DIR == 2 && FIRST && !is_bfp<S> && sizeof(T) == 8 && M >= 16;
+  T pw[PREW ? NW : 1];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     // components one after another: no loads hoisted across them
@@ -112,6 +119,9 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
         if (c == DIR) {
 #pragma unroll
           for (int q = 0; q < NW; ++q) wc[q] = wd[q];
+        } else if (PREW && c == 1) {
+#pragma unroll
+          for (int q = 0; q < (PREW ? NW : 1); ++q) wc[q] = pw[q];
         } else {
           load_window<S, M, H, H, true>(Uc, t, n, wc);
         }
@@ -133,6 +143,9 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
     // solves, whose barriers wait on LDS only.  Neither field is written by
     // this pass.  (not for FP32 z-lines of 32-point chunks, which would spill)
     constexpr bool PF = DIR == 1 || (DIR == 2 && LAST && !(sizeof(T) == 4 && M >= 32));
+    if constexpr (PREW) {
+      if (c == 0) load_window<S, M, H, H, true>(uy, t, n, pw);
+    }
     T a[M];
     if constexpr (PF) {
       if (t.active) {
