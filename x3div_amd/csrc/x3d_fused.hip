@@ -146,7 +146,11 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
     if constexpr (PREW) {
       if (c == 0) load_window<S, M, H, H, true>(uy, t, n, pw);
     }
+    // a last x pass in FP64: the coalesced loads of tmp_c likewise (measured:
+    // FP64 512^3 2.45 -> 2.42 ms; the FP32 batch lost a wave and ran slower)
+    constexpr bool PFX = DIR == 0 && LAST && M % 2 == 0 && !is_bfp<S> && sizeof(T) == 8;
     T a[M];
+    if constexpr (PFX) x_fetch<S, M>(TMc, xb, a);
     if constexpr (PF) {
       if (t.active) {
         const S* src = DIR == 1 ? DUc : TMc;  // (DIR 2: tmp_c)
@@ -174,14 +178,22 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
     } else if constexpr (DIR == 0) {
       // du_c = tmp_c + r_c and the Euler update, through the staging tile:
       // u_c at the chunk points first (the u_x tile, resp. the tile still
-      // holding this component's window load), then tmp_c restaged
-      T uc[M], a[M];
+      // holding this component's window load), then tmp_c restaged (its
+      // loads were issued before the solves where PFX)
+      T uc[M];
       {
         const T* own = (c == 0 ? xd.tile + xd.lane * XPitch<M>::P : xb.tile + xb.lane * XPitch<M>::P);
 #pragma unroll
         for (int k = 0; k < M; ++k) uc[k] = own[k];
       }
-      x_load_chunk<S, M>(TMc, xb, a);
+      if constexpr (PFX) {
+        x_put<S, M>(xb, a);
+        const T* own = xb.tile + xb.lane * XPitch<M>::P;
+#pragma unroll
+        for (int k = 0; k < M; ++k) a[k] = own[k];
+      } else {
+        x_load_chunk<S, M>(TMc, xb, a);
+      }
 #pragma unroll
       for (int k = 0; k < M; ++k) v1[k] = a[k] + v1[k];
       x_store<S, M>(DUc, xb, v1);

@@ -497,6 +497,38 @@ __device__ __forceinline__ void x_load_window(const S* __restrict__ in, const XB
   x_window<S, M, HL, HR>(b, t, win);
 }
 
+// x_stage_in in two halves (M even, not BFP32): the coalesced loads into
+// registers, issued early, and their copy into the tile once it is free.
+template <typename S, int M>
+__device__ __forceinline__ void x_fetch(const S* __restrict__ in, const XBlock<S, M>& b,
+                                        typename Store<S>::T (&r)[M])
+{
+  static_assert(M % 2 == 0 && !is_bfp<S>, "pairs of plain values");
+  using T = typename Store<S>::T;
+  const uint32_t last = b.lim - (uint32_t)sizeof(T);
+#pragma unroll
+  for (int k = 0; k < M / 2; ++k) {
+    const int e = 2 * b.lane + 128 * k;
+    uint32_t off = b.base + (uint32_t)e * (uint32_t)sizeof(T);
+    off = off < last ? off : last;
+    ld2(in, off, r[2 * k], r[2 * k + 1]);
+  }
+}
+template <typename S, int M>
+__device__ __forceinline__ void x_put(const XBlock<S, M>& b, const typename Store<S>::T (&r)[M])
+{
+  constexpr int P = XPitch<M>::P;
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < M / 2; ++k) {
+    const int e = 2 * b.lane + 128 * k;
+    auto* dst = b.tile + (e / M) * P + e % M;
+    dst[0] = r[2 * k];
+    dst[1] = r[2 * k + 1];
+  }
+  wave_lds_sync();
+}
+
 // This lane's chunk of a field, through the tile.
 template <typename S, int M>
 __device__ __forceinline__ void x_load_chunk(const S* __restrict__ in, const XBlock<S, M>& b,
