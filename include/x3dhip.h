@@ -367,7 +367,10 @@ int x3d_step_noalloc(x3d_ctx *ctx, double gdt, double *ux, double *uy, double *u
  * stepped together.  The batch owns the fields: per grid the 10 arrays
  * ux uy uz dux duy duz pp3 px py pz (field index 0-9), of FP64 (precision
  * 8) or FP32 (precision 4) values.  Each step pass is one persistent launch
- * per group of grids sharing a line tiling; Poisson runs per grid.  A grid's
+ * per group of grids sharing a line tiling; Poisson is one launch per stage
+ * and line length over the grids with lines of up to 384 points (the others
+ * solve one by one), and the launches of one pass or stage run side by side
+ * on side streams (X3D_BATCH_PSTREAMS, X3D_BATCH_OVERLAP=0).  A grid's
  * own context (x3d_batch_context) serves the single-grid entry points; while
  * any grid's context holds y-stretching (x3d_set_stretching) or operator
  * tables of the caller's own (x3d_operator_set), x3d_batch_step and
