@@ -81,11 +81,12 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
   const int nth = blockDim.x, tid = threadIdx.x;
   // a first z pass: the u_y window of the second component is loaded during
   // the first component's line solves (its strided rows wait on address
-  // translation).  FP64 16-point chunks only, whose 512-thread work-groups
+  // translation) and formed when it is used.  FP64 arithmetic (FP64 and
+  // BFP32 fields) with 16-point chunks only, whose 512-thread work-groups
   // hold 2 waves per SIMD with or without the window's registers (shorter
-  // chunks would lose occupancy); not for BFP32, whose loads decode at once.
-  constexpr bool PREW = DIR == 2 && FIRST && !is_bfp<S> && sizeof(T) == 8 && M >= 16;
-  T pw[PREW ? NW : 1];
+  // chunks would lose occupancy).
+  constexpr bool PREW = DIR == 2 && FIRST && sizeof(T) == 8 && M >= 16;
+  RawT<S> pw[PREW ? NW : 1];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     // components one after another: no loads hoisted across them
@@ -119,11 +120,15 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
         if (c == DIR) {
 #pragma unroll
           for (int q = 0; q < NW; ++q) wc[q] = wd[q];
-        } else if (PREW && c == 1) {
-#pragma unroll
-          for (int q = 0; q < (PREW ? NW : 1); ++q) wc[q] = pw[q];
         } else {
-          load_window<S, M, H, H, true>(Uc, t, n, wc);
+          bool formed = false;
+          if constexpr (PREW) {
+            if (c == 1) {
+              window_finish<S, M, H, H>(t, pw, wc);
+              formed = true;
+            }
+          }
+          if (!formed) load_window<S, M, H, H, true>(Uc, t, n, wc);
         }
         if constexpr (DIR == 2 && LAST) {
 #pragma unroll
@@ -144,7 +149,7 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
     // this pass.  (not for FP32 z-lines of 32-point chunks, which would spill)
     constexpr bool PF = DIR == 1 || (DIR == 2 && LAST && !(sizeof(T) == 4 && M >= 32));
     if constexpr (PREW) {
-      if (c == 0) load_window<S, M, H, H, true>(uy, t, n, pw);
+      if (c == 0) window_issue<S, M, H, H>(uy, t, n, pw);
     }
     // a last x pass in FP64: the coalesced loads of tmp_c likewise (measured:
     // FP64 512^3 2.45 -> 2.42 ms; the FP32 batch lost a wave and ran slower)
@@ -330,11 +335,17 @@ __device__ __forceinline__ void div_body(const PassArgs<S>& __restrict__ A, int 
       load_window<S, M, 3, 4, DIR != 0>(a, t, n, w);
       rhs_chunk<K_IVP, T, M, 3>(w, cfi, v);
     }
+    // the second window's loads in flight during the first solve; in FP64
+    // arithmetic it is formed (halo shuffles, BFP32 decode) after the solve,
+    // in FP32 before it (FP32's 32-point chunks would lose a wave per SIMD)
     T wb[M + 3];
-    load_window<S, M, 1, 2, DIR != 0>(b, t, n, wb);
+    RawT<S> wbr[sizeof(T) == 8 ? M + 3 : 1];
+    if constexpr (sizeof(T) == 8) window_issue<S, M, 1, 2>(b, t, n, wbr);
+    else load_window<S, M, 1, 2, DIR != 0>(b, t, n, wb);
     __builtin_amdgcn_sched_barrier(0);
     solve_circ<T, M, DIR != 0>(v, t, cci, lds);
     __builtin_amdgcn_sched_barrier(0);
+    if constexpr (sizeof(T) == 8) window_finish<S, M, 1, 2>(t, wbr, wb);
     rhs_chunk<K_DVP, T, M, 1>(wb, cfd, s);
     solve_circ<T, M, DIR != 0>(s, t, ccd, lds);
 #pragma unroll

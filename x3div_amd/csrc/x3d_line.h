@@ -264,6 +264,83 @@ __device__ __forceinline__ LineThread line_thread(const Geom& geo, const Tiling&
   return t;
 }
 
+// Strided y/z windows in two halves, so that a window can be loaded early
+// and formed where it is used: window_issue loads the storage words of the
+// chunk points and, in the lane groups at a wavefront's ends (whose
+// neighbouring chunks live in other wavefronts), of the halo points; the
+// masked loads touch a quarter of the rows.  window_finish decodes them
+// (BFP32; every lane of a 16-lane row takes part, and the halo condition is
+// uniform over a row, whose lanes share a chunk group) and takes the other
+// halo points from the lane groups holding those chunks.
+template <typename S>
+using RawT = typename std::conditional<is_bfp<S>, int, typename Store<S>::T>::type;
+
+template <typename S>
+__device__ __forceinline__ RawT<S> ld_raw(const S* base, uint32_t boff)
+{
+  return ld(reinterpret_cast<const RawT<S>*>(base), boff);
+}
+
+template <typename S>
+__device__ __forceinline__ typename Store<S>::T from_raw(RawT<S> w)
+{
+  if constexpr (is_bfp<S>) return bfp_value(w);
+  else return w;
+}
+
+template <typename S, int M, int HL, int HR>
+__device__ __forceinline__ void window_issue(const S* __restrict__ in, const LineThread& t, int n,
+                                             RawT<S> (&raw)[M + HL + HR])
+{
+#pragma unroll
+  for (int k = 0; k < M; ++k) raw[HL + k] = ld_raw(in, t.at(t.j0 + k));
+#pragma unroll
+  for (int h = 1; h <= HL; ++h) {
+    const int d = (h + M - 1) / M;  // chunks to the left
+    raw[HL - h] = 0;
+    if (t.g < d) {
+      int j = t.j0 - h;
+      j = j < 0 ? j + n : j;
+      raw[HL - h] = ld_raw(in, t.at(j));
+    }
+  }
+#pragma unroll
+  for (int h = 1; h <= HR; ++h) {
+    const int d = (h + M - 1) / M;  // chunks to the right
+    raw[HL + M - 1 + h] = 0;
+    if (t.g + d >= t.G) {
+      int j = t.j0 + M - 1 + h;
+      j = j >= n ? j - n : j;
+      raw[HL + M - 1 + h] = ld_raw(in, t.at(j));
+    }
+  }
+}
+
+template <typename S, int M, int HL, int HR>
+__device__ __forceinline__ void window_finish(const LineThread& t, const RawT<S> (&raw)[M + HL + HR],
+                                              typename Store<S>::T (&win)[M + HL + HR])
+{
+  using T = typename Store<S>::T;
+#pragma unroll
+  for (int k = 0; k < M; ++k) win[HL + k] = from_raw<S>(raw[HL + k]);
+#pragma unroll
+  for (int h = 1; h <= HL; ++h) {
+    const int d = (h + M - 1) / M;  // chunks to the left
+    const int idx = M * d - h;      // index inside that chunk
+    T v = __shfl(win[HL + idx], t.lane_of(t.g >= d ? t.g - d : t.g));
+    if (t.g < d) v = from_raw<S>(raw[HL - h]);
+    win[HL - h] = v;
+  }
+#pragma unroll
+  for (int h = 1; h <= HR; ++h) {
+    const int d = (h + M - 1) / M;  // chunks to the right
+    const int idx = h - 1 - M * (d - 1);
+    T v = __shfl(win[HL + idx], t.lane_of(t.g + d < t.G ? t.g + d : t.g));
+    if (t.g + d >= t.G) v = from_raw<S>(raw[HL + M - 1 + h]);
+    win[HL + M - 1 + h] = v;
+  }
+}
+
 // u at points j0-HL .. j0+M+HR-1 of this thread's line (wrapped mod n).
 // x-lines (consecutive lanes hold consecutive chunks): the M chunk points are
 // loaded from one address with immediate offsets and the halo comes from the
@@ -296,7 +373,9 @@ __device__ __forceinline__ void load_window(const S* __restrict__ in, const Line
     // strided lines: the chunk points from memory; each halo point from the
     // lane group holding that chunk in this wavefront, and from memory only
     // in the groups at the wavefront's ends (their neighbours live in other
-    // wavefronts), so the masked loads touch a quarter of the rows
+    // wavefronts), so the masked loads touch a quarter of the rows (the
+    // values of window_issue + window_finish, here with each halo formed as
+    // soon as its chunk is in, which holds fewer registers)
 #pragma unroll
     for (int k = 0; k < M; ++k) win[HL + k] = sld(in, t.at(t.j0 + k));
 #pragma unroll
