@@ -154,13 +154,15 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
     // a last x pass in FP64: the coalesced loads of tmp_c likewise (measured:
     // FP64 512^3 2.45 -> 2.42 ms; the FP32 batch lost a wave and ran slower)
     constexpr bool PFX = DIR == 0 && LAST && M % 2 == 0 && !is_bfp<S> && sizeof(T) == 8;
+    // (storage words, decoded at use: a BFP32 decode would wait for them here)
     T a[M];
+    RawT<S> ar[M];
     if constexpr (PFX) x_fetch<S, M>(TMc, xb, a);
     if constexpr (PF) {
       if (t.active) {
         const S* src = DIR == 1 ? DUc : TMc;  // (DIR 2: tmp_c)
 #pragma unroll
-        for (int k = 0; k < M; ++k) a[k] = sld(src, t.at(t.j0 + k));
+        for (int k = 0; k < M; ++k) ar[k] = ld_raw(src, t.at(t.j0 + k));
       }
     }
     // keep the two line solves apart: interleaving them doubles the live state
@@ -216,7 +218,7 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
       }
       if (t.active) {
 #pragma unroll
-        for (int k = 0; k < M; ++k) sst(TMc, t.at(t.j0 + k), a[k] + v1[k]);
+        for (int k = 0; k < M; ++k) sst(TMc, t.at(t.j0 + k), from_raw<S>(ar[k]) + v1[k]);
       }
     } else if constexpr (FIRST) {
       if (t.active) {
@@ -225,7 +227,10 @@ __device__ __forceinline__ void transeq_body(const TranseqArgs<S>& __restrict__ 
       }
     } else {
       if (t.active) {
-        if constexpr (!PF) {
+        if constexpr (PF) {
+#pragma unroll
+          for (int k = 0; k < M; ++k) a[k] = from_raw<S>(ar[k]);
+        } else {
 #pragma unroll
           for (int k = 0; k < M; ++k) a[k] = sld(TMc, t.at(t.j0 + k));
         }
